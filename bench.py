@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Throughput of the RSG elastodynamic time step on one MI355X.
+
+Workload: the orthotropic CFRP laminate plate, 256x256x64 cells at 0.05 mm, 16 plies
+[0/45/90/-45]2s, five seeded through-holes (cells removed), node ids seeded-permuted then
+RCM-renumbered, 2 MHz 5-cycle burst on a 2 mm top-face disc, 16 sensors; FP64
+DeviceSolver (the reference's arithmetic, bit-identical results).  One "step" is one
+Model::solve(1): velocity update + stress update + source + sensor recording, with every
+input resident in HBM.
+
+Printed: one JSON line (stdout).  value = Gcell-steps/s of the FP64 DeviceSolver over the
+timed steps; roofline = the stress kernel's algorithmic bytes per launch over its mean
+launch duration (HIP events on the solver's stream, same timed region); cpu_baseline =
+the reference solver itself (oracle/_ref/ref_rsg, built from the reference headers) under
+mpirun on this host's cores, on a bounded sample of the same job; fp32 = the mixed-
+precision form on the same job with its sensor-trace error against FP64.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (each array element touched once per pass), FP64 data model:
+#   stress kernel per cell: C 21x8 + S read+write 6x8x2 + spacings 3x8 + vertices 8x4
+#                           + V gathered 3x8                                  = 344
+#   velocity kernel per node: V read+write 3x8x2 + dt/rho 8 + spacings 3x8 + cell links
+#                           8x4 + S gathered 6x8                              = 160
+#   FP32 form: stress 21x4 + 6x4x2 + 3x4 + 32 + 3x4 = 188; velocity 3x4x2 + 4 + 12 + 32 + 24 = 96
+STRESS_BYTES = {"f64": 344, "f32": 188}
+VELOCITY_BYTES = {"f64": 160, "f32": 96}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    d = np.linalg.norm((a - b).ravel())
+    n = np.linalg.norm(b.ravel())
+    return float(d / n) if n > 0 else float(d)
+
+
+def run_timed(rsg, case, precision, warmup, steps):
+    s = rsg.DeviceSolver.from_case(case, precision)
+    s.solve(warmup)
+    s.synchronize()
+    s.kernel_times()  # reset
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.solve(steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    s.set_timing(False)
+    vms, sms, launches = s.kernel_times()
+    traces = s.traces()
+    V, S = s.get_state()
+    dev_bytes = s.device_bytes()
+    s.close()
+    return dict(seconds=t1 - t0, vel_ms=vms, str_ms=sms, launches=launches, traces=traces, V=V, S=S,
+                dev_bytes=dev_bytes)
+
+
+def cpu_baseline(case, ranks, steps):
+    from oracle import oracle_rsg as orc
+
+    sample = "%s mesh (%d cells), %d steps after 1 warm-up step" % (case.name, case.n_cells, steps)
+    if orc.have_reference():
+        try:
+            _, _, _, _, tim = orc.run_reference(case, nsteps=steps, ranks=ranks, warmup=1, workdir="/tmp")
+            v = case.n_cells * steps / tim["seconds"] / 1e9
+            return dict(value=v, unit="Gcell-steps/s", cores=ranks, kind="reference",
+                        sample=sample + ", reference HostRSGTDUT under mpirun -n %d" % ranks)
+        except Exception as e:  # fall through to the port
+            log("cpu baseline: reference binary failed (%s); timing the oracle port" % e)
+    t0 = time.perf_counter()
+    orc.run_oracle(case, nsteps=steps)
+    dt = time.perf_counter() - t0
+    return dict(value=case.n_cells * steps / dt / 1e9, unit="Gcell-steps/s", cores=1, kind="port",
+                sample=sample + ", oracle restatement, one core (includes its set-up)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-ranks", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: the DeviceSolver holds the whole mesh on one GPU; multi-GPU runs are not part of "
+              "this build (requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from pannde_amd import cases, rsg
+
+    log("bench: building the CFRP plate job (seed %d)" % args.seed)
+    case = cases.cfrp_plate(seed=args.seed, nsteps=args.warmup + args.steps)
+    nc, nn = case.n_cells, case.n_nodes
+    log("bench: %d cells, %d nodes, dt %.4e s" % (nc, nn, case.dt))
+
+    log("bench: FP64 DeviceSolver, %d warm-up + %d timed steps" % (args.warmup, args.steps))
+    r64 = run_timed(rsg, case, rsg.RSG_FP64, args.warmup, args.steps)
+    value = nc * args.steps / r64["seconds"] / 1e9
+    ms_step = r64["seconds"] * 1e3 / args.steps
+    str_avg = r64["str_ms"] / max(1, r64["launches"]) * 1e-3
+    vel_avg = r64["vel_ms"] / max(1, r64["launches"]) * 1e-3
+    str_gbs = STRESS_BYTES["f64"] * nc / str_avg / 1e9
+    vel_gbs = VELOCITY_BYTES["f64"] * nn / vel_avg / 1e9
+    step_gbs = (STRESS_BYTES["f64"] * nc + VELOCITY_BYTES["f64"] * nn) / (str_avg + vel_avg) / 1e9
+    log("bench: FP64 %.3f Gcell-steps/s, %.4f ms/step; stress %.1f us (%.0f GB/s), velocity %.1f us (%.0f GB/s)"
+        % (value, ms_step, str_avg * 1e6, str_gbs, vel_avg * 1e6, vel_gbs))
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "stress_kernel_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch_f64")
+
+    out = {
+        "metric": "Gcell-steps/s FP64 vs FP32 @<=1e-4 trace err; served solves/hr + p50/p99 lat",
+        "value": round(value, 4),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded CFRP plate job generated in-process)",
+        "config": {"workload": "cfrp_plate_256x256x64_dx0.05mm_16ply_5holes_rcm_2MHz", "cells": nc,
+                   "nodes": nn, "seed": args.seed, "solver": "DeviceSolver FP64 (HostRSGTDUT arithmetic)"},
+        "roofline": {"bound": "hbm", "achieved": round(str_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(str_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "stress_kernel<double>", "bytes_per_launch": STRESS_BYTES["f64"] * nc,
+                     "mean_launch_us": round(str_avg * 1e6, 2),
+                     "velocity_kernel": {"achieved": round(vel_gbs, 1), "bytes_per_launch": VELOCITY_BYTES["f64"] * nn,
+                                         "mean_launch_us": round(vel_avg * 1e6, 2)},
+                     "full_step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "device_bytes": r64["dev_bytes"],
+    }
+
+    if not args.no_fp32:
+        log("bench: FP32 DeviceSolver on the same job")
+        r32 = run_timed(rsg, case, rsg.RSG_FP32, args.warmup, args.steps)
+        v32 = nc * args.steps / r32["seconds"] / 1e9
+        tr_err = [rel_l2(r32["traces"][:, s, 2], r64["traces"][:, s, 2]) for s in range(r64["traces"].shape[1])]
+        wf_err = max(rel_l2(r32["V"], r64["V"]), rel_l2(r32["S"], r64["S"]))
+        s32 = r32["str_ms"] / max(1, r32["launches"]) * 1e-3
+        out["fp32"] = {"value": round(v32, 4), "ratio_vs_fp64": round(v32 / value, 3),
+                       "max_trace_rel_l2_vs_fp64": float("%.3e" % max(tr_err)),
+                       "wavefield_rel_l2_vs_fp64": float("%.3e" % wf_err),
+                       "stress_kernel_gbs": round(STRESS_BYTES["f32"] * nc / s32 / 1e9, 1)}
+        log("bench: FP32 %.3f Gcell-steps/s (x%.2f), trace err %.2e, wavefield err %.2e"
+            % (v32, v32 / value, max(tr_err), wf_err))
+
+    if not args.no_cpu_baseline:
+        ranks = args.cpu_ranks or max(1, min(16, os.cpu_count() or 1))
+        log("bench: CPU baseline, reference solver under mpirun -n %d, %d steps" % (ranks, args.cpu_steps))
+        cb = cpu_baseline(case, ranks, args.cpu_steps)
+        out["cpu_baseline"] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in cb.items()}
+        log("bench: CPU baseline %.5f Gcell-steps/s on %d cores (%s)" % (cb["value"], cb["cores"], cb["kind"]))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,191 @@
+/*
+ * pannde_rsg.h -- C ABI of the MI355X RSG DeviceSolver library (libpannde_rsg.so).
+ *
+ * The library is the device half of a drop-in replacement for PanNDE's HostSolver
+ * (reference: modules/HostSolver/include/HostRSGTDUT.hpp, HostUTModel.hpp).  Plain
+ * pointers and sizes only; host arrays are borrowed for the duration of a call, device
+ * buffers are owned by the handle.  One handle owns one HIP stream; calls on one handle
+ * must not overlap.  Every entry point returns RSG_OK (0) or a negative status, and
+ * rsg_last_error() returns the message of the calling thread's last failure (the
+ * reference signals the same conditions with C++ exceptions: std::logic_error("DX") from
+ * RightHex::nodePrep, std::out_of_range from HexMesh::setCell, std::runtime_error).
+ *
+ * Numbering: every array passed in or out uses the caller's node and cell numbering;
+ * the library reorders internally for locality and maps back on the way out.
+ * Stress is in the reference's Voigt order Sxx,Syy,Szz,Syz,Sxz,Sxy
+ * (modules/PanNDE/include/StandardNames.hpp:167).
+ */
+#ifndef PANNDE_RSG_H
+#define PANNDE_RSG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSG_ABI_VERSION 1
+
+/* status codes */
+#define RSG_OK 0
+#define RSG_ERR_ARG (-1)    /* bad argument (null pointer, size, index out of range) */
+#define RSG_ERR_MESH (-2)   /* RightHex::nodePrep non-positive node spacing ("DX"/"DY"/"DZ") */
+#define RSG_ERR_DEVICE (-3) /* HIP runtime failure */
+#define RSG_ERR_STATE (-4)  /* call not valid in the handle's current state */
+#define RSG_ERR_NOMEM (-5)  /* device memory exhausted / over budget */
+
+/* field placement, PanNDE::Field::FieldType (modules/PanNDE/include/Field.hpp) */
+#define RSG_FIELD_NODE 0
+#define RSG_FIELD_CELL 1
+#define RSG_FIELD_CONSTANT 2
+
+/* arithmetic of a handle */
+#define RSG_FP64 0 /* FP64 fields and updates in the reference's operation order */
+#define RSG_FP32 1 /* FP32 fields and updates; coefficients precomputed in FP64 then
+                      rounded; sensor traces and energy accumulated in FP64 */
+
+/* source channel kinds */
+#define RSG_FORCE 0  /* body force in the velocity update, HostRSGTDUT::evalXds */
+#define RSG_MOMENT 1 /* stress source in the stress update, HostUTModel::evalMoments */
+
+typedef struct rsg_instance* rsg_handle;
+
+/* Hexahedral mesh, the data of PanNDE::Mesh (Mesh.hpp:74-248) / HostData::HexMesh:
+ * node coordinates and per-cell vertex lists in CGNS order
+ * [000,100,110,010,001,101,111,011].  Node-to-cell links are derived inside the
+ * library with HexMesh::buildLinks' slot rule (HexMesh.hpp:300-313). */
+typedef struct {
+  int32_t n_nodes;
+  int32_t n_cells;
+  const double* x;
+  const double* y;
+  const double* z;
+  const int32_t* cell_nodes; /* [n_cells][8] */
+} rsg_mesh_desc;
+
+/* A material field as PanNDE::Field exposes it: data() and type().  Length is n_nodes
+ * (NODE), n_cells (CELL) or 1 (CONSTANT).  Read with HostField::atNode / atCell
+ * semantics (HostField.hpp:121-154). */
+typedef struct {
+  const double* data;
+  int32_t type;
+} rsg_field_desc;
+
+/* Model constants: the domain FieldBundle of HostRSGTDUT/HostUTModel (density and
+ * C11..C66 under the ElasticMaterialNames, StandardNames.hpp:75-147) plus dt. */
+typedef struct {
+  double dt;
+  rsg_field_desc density;
+  rsg_field_desc cij[21]; /* C11,C12,C13,C14,C15,C16,C22,..,C26,C33,..,C36,C44,C45,C46,
+                             C55,C56,C66 */
+  int32_t moment_form;    /* 0: HostRSGTDUT update; 1: HostUTModel update (stress
+                             sources allowed) */
+} rsg_model_desc;
+
+/* One source channel: one non-null MultiVariate of one transducer row (force) or one
+ * moment-tensor row (moment).  Channels of one kind and component are summed in the
+ * order given, which must be the reference's row order.
+ *   FORCE : support = cells whose centre the source covers (its checkAt at
+ *           HostRSGTDUT::cellcenter); a node's term list walks its 8 connected-cell
+ *           slots, then the channels, exactly as evalXds does.
+ *   MOMENT: support = nodes the source covers (checkAt at the node coordinate); a
+ *           cell's term list walks its 8 vertices, then the channels (evalMoments).
+ * Each term adds the channel's per-step amplitude, i.e. 0.125*evalAt(point, t).
+ * If n_signal > 0 the library evaluates it itself as 0.125*(s(t)*1.0) with s the
+ * HostData::HostUnivariate table (signal_t, signal_v) (HostUnivariate.hpp:109-142);
+ * otherwise amplitudes come with every rsg_step call. */
+typedef struct {
+  int32_t kind;
+  int32_t component; /* FORCE: direction 0..2; MOMENT: Voigt index 0..5 */
+  int32_t n_support;
+  const int32_t* support;
+  int32_t n_signal;
+  const double* signal_t;
+  const double* signal_v;
+} rsg_channel_desc;
+
+/* Replaces HostRSGTDUT::HostRSGTDUT / HostUTModel::HostUTModel set-up
+ * (HostRSGTDUT.hpp:139-146,180-251; HostUTModel.hpp:113-124): spacings (RightHex
+ * nodePrep/cellPrep, RightHex.hpp:253-309), density at nodes and C at cells, the
+ * locality reordering, and the device upload.  The state starts at zero and time at 0
+ * (the drivers' zero initial condition, TransducerModel.cpp:150-159). */
+int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t precision,
+               rsg_handle* out);
+
+/* Replaces the transducer list of HostRSGTDUT (setupTransducers, HostRSGTDUT.hpp:213-219)
+ * and HostUTModel::applyTransducer / applyMomentExcitation (HostUTModel.hpp:138-170).
+ * Replaces any previous channel set. */
+int rsg_set_channels(rsg_handle h, int32_t n_channels, const rsg_channel_desc* channels);
+
+/* Sensor nodes whose (Vx,Vy,Vz) are recorded after every step, fused into the velocity
+ * kernel (the velocity table of TransducerModel.cpp:106-149).  Replaces any previous set
+ * and clears recorded traces. */
+int rsg_set_sensors(rsg_handle h, int32_t n_sensors, const int32_t* nodes);
+
+/* Replaces Model::solve(nsteps) (HostRSGTDUT.hpp:162-169): nsteps of
+ * {velocity update; stress update; time += dt}.  amplitudes: NULL when every channel
+ * carries a signal table, else [nsteps][n_channels] values 0.125*evalAt(point, t_k),
+ * t_k being the time at the start of step k.  *time_out (if not NULL) receives the
+ * time after the last step, as solve() returns it. */
+int rsg_step(rsg_handle h, int32_t nsteps, const double* amplitudes, double* time_out);
+
+/* Replaces Model::getStates() (HostRSGTDUT.hpp:153): copies the live state out.
+ * V: [3][n_nodes] (Vx,Vy,Vz); S: [6][n_cells] in Voigt order.  Either may be NULL. */
+int rsg_get_state(rsg_handle h, double* V, double* S);
+
+/* Writes a state (e.g. an initial condition other than zero).  Either may be NULL. */
+int rsg_set_state(rsg_handle h, const double* V, const double* S);
+
+/* Recorded sensor velocities for steps [first_step, first_step+nsteps) counted from the
+ * last rsg_set_sensors: out[step][sensor][3]. */
+int rsg_get_traces(rsg_handle h, int64_t first_step, int64_t nsteps, double* out);
+
+/* Blocks until every step queued on the handle's stream has finished (rsg_step only
+ * enqueues work).  The reference's solve() is synchronous; this is its completion
+ * point. */
+int rsg_synchronize(rsg_handle h);
+
+/* Current simulation time (the value solve() last returned) and steps taken. */
+int rsg_get_time(rsg_handle h, double* time, int64_t* steps);
+
+/* Discrete energy of the current state, accumulated in FP64:
+ * kinetic = sum_n 0.5*rho_n*|V_n|^2*w_n with w_n = 8*dxn*dyn*dzn, and
+ * stress  = sum_c 0.5*|S_c|^2*dxc*dyc*dzc (a norm of the stress field, no compliance). */
+int rsg_energy(rsg_handle h, double* kinetic, double* stress_norm);
+
+/* Device-kernel timing: when enabled, every velocity/stress launch is bracketed by HIP
+ * events on the handle's stream; totals accumulate until read (and reset). */
+int rsg_set_timing(rsg_handle h, int32_t enabled);
+int rsg_get_kernel_times(rsg_handle h, double* velocity_ms, double* stress_ms,
+                         int64_t* launches);
+
+/* Bytes of device memory the handle holds (fields, coefficients, connectivity, source
+ * and trace buffers). */
+int rsg_device_bytes(rsg_handle h, int64_t* bytes);
+
+/* Releases the handle and its device memory. */
+int rsg_destroy(rsg_handle h);
+
+/* Copies the calling thread's last error message into buf (NUL-terminated); returns its
+ * length. */
+int rsg_last_error(char* buf, size_t len);
+
+/* HostNormalXDZ::checkAt (modules/Stubs/include/HostNormalXDZ.hpp:140-145) over a mesh:
+ * at = 0 tests every cell centre (HostRSGTDUT::cellcenter), at = 1 every node.  Writes
+ * the ids that pass, ascending, to ids (capacity n_cells or n_nodes) and their count to
+ * *count. */
+int rsg_disc_support(const rsg_mesh_desc* mesh, int32_t at, const double center[3],
+                     double radius, double ds, int32_t* ids, int32_t* count);
+
+/* HostUnivariate::evalAt (HostUnivariate.hpp:109-121) of a (t, v) table at n_eval
+ * times (keys sorted, a repeated key keeps its first value).  A time below the first
+ * key is undefined in the reference and returns RSG_ERR_ARG here. */
+int rsg_signal_eval(int32_t n, const double* t, const double* v, int32_t n_eval,
+                    const double* at, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PANNDE_RSG_H */

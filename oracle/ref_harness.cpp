@@ -1,0 +1,393 @@
+// ref_harness.cpp -- runs the reference's own header-only RSG solver on a case file.
+//
+// TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline, never the product).
+// oracle/Makefile compiles this driver against the reference headers where they lie
+// (modules/HostSolver, HostData, Stubs, PanNDE, NetMPI) into oracle/_ref/ref_rsg; nothing
+// of the reference is copied.  The driver does what applications/source/CaseRunners/
+// TransducerModel.cpp:46-83 and WaterColumnModel.cpp:85-114 do, minus VTK and METIS:
+//   * the mesh comes from a case file (written by pannde_amd.cases / the case-file writer)
+//     and is built through HostData::HexMeshFactory (buildLinks runs inside HexMesh);
+//   * density and C11..C66 become HostField's of the stated NODE/CELL/CONSTANT type;
+//   * sources are Stubs::HostNormalXDZ over HostData::HostUnivariate, one transducer row
+//     per listed force entry (HostRSGTDUT) or applyTransducer / applyMomentExcitation
+//     rows (HostUTModel), exactly as ArbSigTransducerManager builds them (Boss.hpp:502-513);
+//   * the communicator is NetMPI::MPICommunicator.  Under mpirun -n P the nodes are split
+//     into x-slabs; each rank keeps every cell touching an owned node (the 1-ring rule of
+//     MetisHexPartitioner.hpp:168-238) with local ids ascending in global id, so the
+//     reference halo exchange (MPIHaloExchanger.hpp) pairs its send/recv sets correctly.
+// The step loop is the driver's: time = model->solve(); comm->broadcastValue(&time).
+//
+// Usage: ref_rsg CASE OUT [--warmup W]   (steps and sensors come from the case file)
+// Prints "REF_TIMING ranks=P steps=K seconds=T setup_seconds=S" on rank 0.
+#include <mpi.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "Array.hpp"
+#include "Field.hpp"
+#include "Mesh.hpp"
+#include "Model.hpp"
+#include "MultiVariate.hpp"
+#include "StandardNames.hpp"
+#include "Univariate.hpp"
+
+#include "HexMesh.hpp"
+#include "HostArray.hpp"
+#include "HostField.hpp"
+#include "HostUnivariate.hpp"
+#include "HostNormalXDZ.hpp"
+#include "MPICommunicator.hpp"
+#include "HostRSGTDUT.hpp"
+#include "HostUTModel.hpp"
+
+namespace {
+
+struct Disc {
+  int32_t comp;
+  double center[3], radius, ds;
+  std::vector<double> t, v;
+};
+
+struct FieldIn {
+  int32_t type;
+  std::vector<double> data;
+};
+
+struct Case {
+  int32_t nn = 0, nc = 0;
+  std::vector<double> x, y, z;
+  std::vector<int32_t> cells;
+  FieldIn rho;
+  FieldIn c[21];
+  double dt = 0;
+  int32_t moment_form = 0;
+  std::vector<Disc> force, moment;
+  std::vector<int32_t> sens;
+  int32_t nsteps = 0;
+  std::vector<double> V0, S0;
+};
+
+struct Reader {
+  FILE* f;
+  template <class T>
+  T get() {
+    T v;
+    if (fread(&v, sizeof(T), 1, f) != 1) throw std::runtime_error("case file truncated");
+    return v;
+  }
+  template <class T>
+  void vec(std::vector<T>& out, size_t n) {
+    out.resize(n);
+    if (n && fread(out.data(), sizeof(T), n, f) != n) throw std::runtime_error("case file truncated");
+  }
+};
+
+Case readCase(const char* path) {
+  FILE* f = fopen(path, "rb");
+  if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+  Reader r{f};
+  char magic[8];
+  if (fread(magic, 1, 8, f) != 8 || memcmp(magic, "RSGCASE1", 8) != 0)
+    throw std::runtime_error("not an RSGCASE1 file");
+  Case cs;
+  cs.nn = r.get<int32_t>();
+  cs.nc = r.get<int32_t>();
+  r.vec(cs.x, cs.nn);
+  r.vec(cs.y, cs.nn);
+  r.vec(cs.z, cs.nn);
+  r.vec(cs.cells, (size_t)cs.nc * 8);
+  auto readField = [&](FieldIn& fi) {
+    fi.type = r.get<int32_t>();
+    size_t len = fi.type == 0 ? cs.nn : (fi.type == 1 ? cs.nc : 1);
+    r.vec(fi.data, len);
+  };
+  readField(cs.rho);
+  for (int k = 0; k < 21; k++) readField(cs.c[k]);
+  cs.dt = r.get<double>();
+  cs.moment_form = r.get<int32_t>();
+  for (int pass = 0; pass < 2; pass++) {
+    auto& list = pass ? cs.moment : cs.force;
+    int32_t n = r.get<int32_t>();
+    list.resize(n);
+    for (auto& d : list) {
+      d.comp = r.get<int32_t>();
+      for (int k = 0; k < 3; k++) d.center[k] = r.get<double>();
+      d.radius = r.get<double>();
+      d.ds = r.get<double>();
+      int32_t ns = r.get<int32_t>();
+      r.vec(d.t, ns);
+      r.vec(d.v, ns);
+    }
+  }
+  int32_t ns = r.get<int32_t>();
+  r.vec(cs.sens, ns);
+  cs.nsteps = r.get<int32_t>();
+  int32_t has_init = r.get<int32_t>();
+  if (has_init) {
+    r.vec(cs.V0, (size_t)3 * cs.nn);
+    r.vec(cs.S0, (size_t)6 * cs.nc);
+  }
+  fclose(f);
+  return cs;
+}
+
+std::shared_ptr<PanNDE::Array<double>> makeF64Array(const std::vector<double>& v) {
+  auto a = std::make_shared<HostData::HostArray<double>>();
+  a->resize(v.size());
+  for (size_t k = 0; k < v.size(); k++) a->at(k) = v[k];
+  return a;
+}
+
+std::shared_ptr<PanNDE::MultiVariate> makeDisc(const Disc& d) {
+  auto sig = HostData::HostUnivariate::makeShared(makeF64Array(d.t), makeF64Array(d.v));
+  double center[3] = {d.center[0], d.center[1], d.center[2]};
+  return Stubs::HostNormalXDZ::makeShared(center, d.radius, d.ds, sig);
+}
+
+const char* kCijNames[21] = {"C11", "C12", "C13", "C14", "C15", "C16", "C22",
+                             "C23", "C24", "C25", "C26", "C33", "C34", "C35",
+                             "C36", "C44", "C45", "C46", "C55", "C56", "C66"};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  MPI_Init(&argc, &argv);
+  int rank = 0, nranks = 1;
+  MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+  MPI_Comm_size(MPI_COMM_WORLD, &nranks);
+  if (argc < 3) {
+    if (rank == 0) fprintf(stderr, "usage: ref_rsg CASE OUT [--warmup W]\n");
+    MPI_Finalize();
+    return 2;
+  }
+  int warmup = 0;
+  for (int k = 3; k + 1 < argc; k++)
+    if (std::string(argv[k]) == "--warmup") warmup = atoi(argv[k + 1]);
+
+  double t_setup0 = MPI_Wtime();
+  Case cs = readCase(argv[1]);
+
+  // --- x-slab node ownership and the 1-ring local mesh of this rank
+  double xmin = cs.x.empty() ? 0 : *std::min_element(cs.x.begin(), cs.x.end());
+  double xmax = cs.x.empty() ? 0 : *std::max_element(cs.x.begin(), cs.x.end());
+  std::vector<int32_t> owner(cs.nn, 0);
+  if (nranks > 1 && xmax > xmin)
+    for (int32_t n = 0; n < cs.nn; n++)
+      owner[n] = std::min(nranks - 1, (int)std::floor((cs.x[n] - xmin) / (xmax - xmin) * nranks));
+  std::vector<int32_t> lcells;
+  for (int32_t c = 0; c < cs.nc; c++)
+    for (int v = 0; v < 8; v++)
+      if (owner[cs.cells[(size_t)c * 8 + v]] == rank) {
+        lcells.push_back(c);
+        break;
+      }
+  std::vector<int32_t> g2l(cs.nn, -1), lnodes;
+  {
+    std::vector<char> used(cs.nn, 0);
+    for (int32_t c : lcells)
+      for (int v = 0; v < 8; v++) used[cs.cells[(size_t)c * 8 + v]] = 1;
+    for (int32_t n = 0; n < cs.nn; n++)
+      if (used[n]) {
+        g2l[n] = (int32_t)lnodes.size();
+        lnodes.push_back(n);
+      }
+  }
+  auto meshMaker = HostData::HexMeshFactory::makeShared();
+  std::vector<PanNDE::Mesh::Node> nodes;
+  std::vector<PanNDE::Mesh::Cell> cells;
+  nodes.reserve(lnodes.size());
+  cells.reserve(lcells.size());
+  for (int32_t n : lnodes) {
+    double p[3] = {cs.x[n], cs.y[n], cs.z[n]};
+    nodes.push_back(meshMaker->makeNode(p, n, owner[n]));
+  }
+  for (int32_t c : lcells) {
+    int32_t box[8];
+    for (int v = 0; v < 8; v++) box[v] = g2l[cs.cells[(size_t)c * 8 + v]];
+    cells.push_back(meshMaker->makeCell(box, 8, c, owner[cs.cells[(size_t)c * 8]]));
+  }
+  std::shared_ptr<PanNDE::Mesh> mesh =
+      meshMaker->makeManagedMesh(nodes.data(), (int64_t)nodes.size(), cells.data(),
+                                 (int64_t)cells.size(), rank);
+  nodes.clear();
+  cells.clear();
+
+  // --- domain (material) and solution bundles, local slices of the global inputs
+  PanNDE::ElasticMaterialNames mat;
+  PanNDE::ElasticStateNames st;
+  auto domain = HostData::HostFieldBundle::makeShared();
+  domain->mesh() = mesh;
+  auto addField = [&](std::shared_ptr<PanNDE::FieldBundle> b, const std::string& name,
+                      const FieldIn& fi) {
+    auto type = (PanNDE::Field::FieldType)fi.type;
+    b->emplaceField(name, type);
+    auto f = b->field(name);
+    if (fi.type == 0)
+      for (size_t k = 0; k < lnodes.size(); k++) f->at((int)k) = fi.data[lnodes[k]];
+    else if (fi.type == 1)
+      for (size_t k = 0; k < lcells.size(); k++) f->at((int)k) = fi.data[lcells[k]];
+    else
+      f->at(0) = fi.data[0];
+  };
+  addField(domain, mat.density, cs.rho);
+  for (int k = 0; k < 21; k++) addField(domain, kCijNames[k], cs.c[k]);
+
+  auto solution = HostData::HostFieldBundle::makeShared();
+  solution->mesh() = mesh;
+  for (int kd = 0; kd < 3; kd++) solution->emplaceField(st.V[kd], PanNDE::Field::NODE);
+  for (int ki = 0; ki < 3; ki++)
+    for (int kj = ki; kj < 3; kj++) solution->emplaceField(st.S[ki][kj], PanNDE::Field::CELL);
+  if (!cs.V0.empty()) {
+    for (int kd = 0; kd < 3; kd++) {
+      auto f = solution->field(st.V[kd]);
+      for (size_t k = 0; k < lnodes.size(); k++)
+        f->at((int)k) = cs.V0[(size_t)kd * cs.nn + lnodes[k]];
+    }
+    for (int ks = 0; ks < 6; ks++) {
+      auto f = solution->field(st.SI[ks]);
+      for (size_t k = 0; k < lcells.size(); k++)
+        f->at((int)k) = cs.S0[(size_t)ks * cs.nc + lcells[k]];
+    }
+  }
+
+  // --- communicator and model
+  std::shared_ptr<PanNDE::Communicator> comm = NetMPI::MPICommunicator::makeShared();
+  std::shared_ptr<PanNDE::Model> model;
+  if (!cs.moment_form) {
+    if (!cs.moment.empty()) throw std::runtime_error("HostRSGTDUT has no moment sources");
+    std::vector<std::shared_ptr<PanNDE::MultiVariate>> flat(3 * std::max<size_t>(1, cs.force.size()));
+    for (size_t i = 0; i < cs.force.size(); i++) flat[3 * i + cs.force[i].comp] = makeDisc(cs.force[i]);
+    auto xds = (std::shared_ptr<PanNDE::MultiVariate>(*)[3])flat.data();
+    model = std::make_shared<HostSolver::HostRSGTDUT>(cs.dt, domain, solution, xds,
+                                                      (int)cs.force.size(), comm);
+  } else {
+    auto ut = std::make_shared<HostSolver::HostUTModel>(cs.dt, domain, solution, comm);
+    for (auto& d : cs.force) {
+      std::shared_ptr<PanNDE::MultiVariate> row[3] = {nullptr, nullptr, nullptr};
+      row[d.comp] = makeDisc(d);
+      ut->applyTransducer(row);
+    }
+    for (auto& d : cs.moment) {
+      std::shared_ptr<PanNDE::MultiVariate> row[6] = {nullptr, nullptr, nullptr,
+                                                      nullptr, nullptr, nullptr};
+      row[d.comp] = makeDisc(d);
+      ut->applyMomentExcitation(row);
+    }
+    model = ut;
+  }
+  auto states = model->getStates();
+  std::shared_ptr<PanNDE::Field> Vf[3], Sf[6];
+  for (int kd = 0; kd < 3; kd++) Vf[kd] = states->field(st.V[kd]);
+  for (int ks = 0; ks < 6; ks++) Sf[ks] = states->field(st.SI[ks]);
+
+  // sensors owned here: (sensor slot, local node)
+  std::vector<std::pair<int32_t, int32_t>> mysens;
+  for (size_t s = 0; s < cs.sens.size(); s++) {
+    int32_t g = cs.sens[s];
+    if (owner[g] == rank && g2l[g] >= 0) mysens.push_back({(int32_t)s, g2l[g]});
+  }
+  std::vector<double> mytraces((size_t)cs.nsteps * mysens.size() * 3);
+  double t_setup = MPI_Wtime() - t_setup0;
+
+  // --- the driver's step loop (TransducerModel.cpp:68-73)
+  double time = 0;
+  for (int k = 0; k < warmup; k++) {
+    time = model->solve();
+    comm->broadcastValue(&time);
+  }
+  MPI_Barrier(MPI_COMM_WORLD);
+  double t0 = MPI_Wtime();
+  for (int k = 0; k < cs.nsteps; k++) {
+    time = model->solve();
+    comm->broadcastValue(&time);
+    for (size_t s = 0; s < mysens.size(); s++)
+      for (int d = 0; d < 3; d++)
+        mytraces[((size_t)k * mysens.size() + s) * 3 + d] = Vf[d]->at(mysens[s].second);
+  }
+  MPI_Barrier(MPI_COMM_WORLD);
+  double t_loop = MPI_Wtime() - t0;
+
+  // --- gather owned values on rank 0
+  std::vector<int32_t> myn, myc;
+  for (size_t k = 0; k < lnodes.size(); k++)
+    if (owner[lnodes[k]] == rank) myn.push_back((int32_t)k);
+  for (size_t k = 0; k < lcells.size(); k++)
+    if (owner[cs.cells[(size_t)lcells[k] * 8]] == rank) myc.push_back((int32_t)k);
+  std::vector<double> V((size_t)3 * cs.nn), S((size_t)6 * cs.nc),
+      traces((size_t)cs.nsteps * cs.sens.size() * 3);
+  auto packAndSend = [&](int dest) {
+    std::vector<int32_t> ids;
+    std::vector<double> vals;
+    for (int32_t k : myn) {
+      ids.push_back(lnodes[k]);
+      for (int d = 0; d < 3; d++) vals.push_back(Vf[d]->at(k));
+    }
+    int32_t nnode = (int32_t)myn.size();
+    for (int32_t k : myc) {
+      ids.push_back(lcells[k]);
+      for (int s = 0; s < 6; s++) vals.push_back(Sf[s]->at(k));
+    }
+    for (auto& ps : mysens) ids.push_back(ps.first);
+    int32_t hdr[3] = {nnode, (int32_t)myc.size(), (int32_t)mysens.size()};
+    if (dest == rank) return std::make_pair(ids, vals);
+    MPI_Send(hdr, 3, MPI_INT, dest, 1, MPI_COMM_WORLD);
+    MPI_Send(ids.data(), (int)ids.size(), MPI_INT, dest, 2, MPI_COMM_WORLD);
+    MPI_Send(vals.data(), (int)vals.size(), MPI_DOUBLE, dest, 3, MPI_COMM_WORLD);
+    MPI_Send(mytraces.data(), (int)mytraces.size(), MPI_DOUBLE, dest, 4, MPI_COMM_WORLD);
+    return std::make_pair(ids, vals);
+  };
+  auto unpack = [&](const int32_t* hdr, const std::vector<int32_t>& ids,
+                    const std::vector<double>& vals, const std::vector<double>& tr) {
+    size_t iv = 0, ii = 0;
+    for (int32_t k = 0; k < hdr[0]; k++, ii++)
+      for (int d = 0; d < 3; d++) V[(size_t)d * cs.nn + ids[ii]] = vals[iv++];
+    for (int32_t k = 0; k < hdr[1]; k++, ii++)
+      for (int s = 0; s < 6; s++) S[(size_t)s * cs.nc + ids[ii]] = vals[iv++];
+    for (int32_t k = 0; k < hdr[2]; k++, ii++)
+      for (int32_t st_ = 0; st_ < cs.nsteps; st_++)
+        for (int d = 0; d < 3; d++)
+          traces[((size_t)st_ * cs.sens.size() + ids[ii]) * 3 + d] =
+              tr[((size_t)st_ * hdr[2] + k) * 3 + d];
+  };
+  if (rank == 0) {
+    auto mine = packAndSend(0);
+    int32_t hdr[3] = {(int32_t)myn.size(), (int32_t)myc.size(), (int32_t)mysens.size()};
+    unpack(hdr, mine.first, mine.second, mytraces);
+    for (int src = 1; src < nranks; src++) {
+      int32_t h[3];
+      MPI_Recv(h, 3, MPI_INT, src, 1, MPI_COMM_WORLD, MPI_STATUS_IGNORE);
+      std::vector<int32_t> ids(h[0] + h[1] + h[2]);
+      std::vector<double> vals((size_t)3 * h[0] + (size_t)6 * h[1]);
+      std::vector<double> tr((size_t)cs.nsteps * h[2] * 3);
+      MPI_Recv(ids.data(), (int)ids.size(), MPI_INT, src, 2, MPI_COMM_WORLD, MPI_STATUS_IGNORE);
+      MPI_Recv(vals.data(), (int)vals.size(), MPI_DOUBLE, src, 3, MPI_COMM_WORLD, MPI_STATUS_IGNORE);
+      MPI_Recv(tr.data(), (int)tr.size(), MPI_DOUBLE, src, 4, MPI_COMM_WORLD, MPI_STATUS_IGNORE);
+      unpack(h, ids, vals, tr);
+    }
+    FILE* f = fopen(argv[2], "wb");
+    if (!f) throw std::runtime_error("cannot write output");
+    int32_t h[4] = {cs.nn, cs.nc, cs.nsteps, (int32_t)cs.sens.size()};
+    fwrite("RSGOUT01", 1, 8, f);
+    fwrite(h, sizeof(int32_t), 4, f);
+    fwrite(&time, sizeof(double), 1, f);
+    fwrite(V.data(), sizeof(double), V.size(), f);
+    fwrite(S.data(), sizeof(double), S.size(), f);
+    fwrite(traces.data(), sizeof(double), traces.size(), f);
+    fclose(f);
+    printf("REF_TIMING ranks=%d steps=%d seconds=%.9f setup_seconds=%.3f cells=%d nodes=%d\n",
+           nranks, cs.nsteps, t_loop, t_setup, cs.nc, cs.nn);
+    fflush(stdout);
+  } else {
+    packAndSend(0);
+  }
+  comm->Finalize();
+  return 0;
+}

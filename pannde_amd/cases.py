@@ -1,0 +1,306 @@
+"""Synthetic RSG jobs: hexahedral meshes, materials, sources and sensors.
+
+The conventions follow the reference's case makers and drivers (paths relative to the
+reference tree):
+
+* node numbering loops x outermost, then y, then z innermost, and cells are emitted in
+  the same loop order with vertices in CGNS order [000,100,110,010,001,101,111,011]
+  (py_utils/Case_Makers/Composite_Laminate_Transducer_Case_Maker.py:138-152;
+  modules/Stubs/include/PlateBuilder.hpp:112-170);
+* density and C11..C66 are CELL arrays (upper-triangle order C11,C12,..,C66, the
+  ``combinations_with_replacement`` order of the case makers, :160-172);
+* a ply stiffness is rotated about z as a fourth-order tensor (rotateStiffnessMatrix,
+  :70-83);
+* the source signal is the case makers' windowed tone burst: t = arange(0, steps*dt, dt),
+  sin(2 pi f t + phase) * hann(steps), normalised to a maximum of 1 (:193-201);
+* dt is a fraction of ``1/(c (1/dx+1/dy+1/dz))`` (py_utils/ParameterCalc.py:80);
+* the transducer is HostNormalXDZ(center, radius, dz of cell 0, signal) driving the z
+  force (applications/include/Boss.hpp:502-513,530-536;
+  applications/source/CaseRunners/TransducerModel.cpp:185-198).
+
+Nothing here touches a GPU; the arrays feed ``pannde_amd.rsg.DeviceSolver``.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+FIELD_NODE, FIELD_CELL, FIELD_CONSTANT = 0, 1, 2
+
+# (i, j) of the 21 stored stiffness entries, 0-based Voigt indices.
+CIJ_PAIRS = [(i, j) for i in range(6) for j in range(i, 6)]
+CIJ_NAMES = ["C%d%d" % (i + 1, j + 1) for i, j in CIJ_PAIRS]
+
+
+@dataclasses.dataclass
+class Disc:
+    """HostNormalXDZ: circular window of ``radius`` around ``center`` and
+    |z - zc| <= 0.75*ds, times a HostUnivariate signal (t, v)."""
+
+    component: int
+    center: Sequence[float]
+    radius: float
+    ds: float
+    signal_t: np.ndarray
+    signal_v: np.ndarray
+
+
+@dataclasses.dataclass
+class Field:
+    type: int
+    data: np.ndarray
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    cells: np.ndarray  # int32 [nc, 8]
+    rho: Field
+    cij: List[Field]  # 21 entries
+    dt: float
+    nsteps: int
+    force: List[Disc] = dataclasses.field(default_factory=list)
+    moment: List[Disc] = dataclasses.field(default_factory=list)
+    moment_form: int = 0
+    sensors: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
+    seed: int = 0
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def n_cells(self) -> int:
+        return int(self.cells.shape[0])
+
+
+# ---------------------------------------------------------------- meshes
+
+def structured_mesh(nx, ny, nz, dx, dy, dz, keep=None, origin=(0.0, 0.0, 0.0)):
+    """Right-hex grid of nx*ny*nz cells in case-maker numbering.
+
+    ``keep`` (bool [nx, ny, nz]) removes cells; nodes that no kept cell uses are dropped
+    and the rest renumbered in the same loop order.  Coordinates are ``origin + k*d``.
+    Returns (x, y, z, cells, cell_ijk)."""
+    if keep is None:
+        keep = np.ones((nx, ny, nz), dtype=bool)
+    keep = np.asarray(keep, dtype=bool)
+    used = np.zeros((nx + 1, ny + 1, nz + 1), dtype=bool)
+    for di in (0, 1):
+        for dj in (0, 1):
+            for dk in (0, 1):
+                used[di:di + nx, dj:dj + ny, dk:dk + nz] |= keep
+    nid = np.full(used.shape, -1, dtype=np.int64)
+    nid[used] = np.arange(int(used.sum()))  # C order: x outer, z inner
+    ii, jj, kk = np.nonzero(used)
+    x = origin[0] + ii * dx
+    y = origin[1] + jj * dy
+    z = origin[2] + kk * dz
+    ci, cj, ck = np.nonzero(keep)
+    offs = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    cells = np.empty((ci.shape[0], 8), dtype=np.int32)
+    for v, (a, b, c) in enumerate(offs):
+        cells[:, v] = nid[ci + a, cj + b, ck + c]
+    return (x.astype(np.float64), y.astype(np.float64), z.astype(np.float64), cells,
+            np.stack([ci, cj, ck], axis=1))
+
+
+def renumber_nodes(x, y, z, cells, new_of_old):
+    """Applies a node permutation: node ``o`` becomes ``new_of_old[o]``."""
+    inv = np.empty_like(new_of_old)
+    inv[new_of_old] = np.arange(new_of_old.shape[0])
+    return x[inv], y[inv], z[inv], new_of_old[cells].astype(np.int32)
+
+
+def rcm_order(n_nodes, cells):
+    """Reverse Cuthill-McKee order (scipy) of the graph whose edges are the 12 hex
+    edges; returns new_of_old."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+
+    e = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6), (3, 7)]
+    a = np.concatenate([cells[:, i] for i, _ in e]).astype(np.int64)
+    b = np.concatenate([cells[:, j] for _, j in e]).astype(np.int64)
+    g = sp.coo_matrix((np.ones(2 * a.shape[0], dtype=np.int8), (np.concatenate([a, b]), np.concatenate([b, a]))),
+                      shape=(n_nodes, n_nodes)).tocsr()
+    perm = reverse_cuthill_mckee(g, symmetric_mode=True)  # perm[new] = old
+    new_of_old = np.empty(n_nodes, dtype=np.int64)
+    new_of_old[perm] = np.arange(n_nodes)
+    return new_of_old
+
+
+# ---------------------------------------------------------------- materials
+
+def isotropic_c(lam, mu):
+    c = np.zeros((6, 6))
+    c[:3, :3] = lam
+    for i in range(3):
+        c[i, i] = lam + 2 * mu
+        c[i + 3, i + 3] = mu
+    return c
+
+
+_VOIGT = np.array([[0, 5, 4], [5, 1, 3], [4, 3, 2]])
+
+
+def rotate_c_about_z(c6, degrees):
+    """C'_ijkl = R_ia R_jb R_kc R_ld C_abcd with R a rotation by ``degrees`` about z."""
+    c4 = c6[_VOIGT[:, :, None, None], _VOIGT[None, None, :, :]]
+    a = math.radians(degrees)
+    r = np.array([[math.cos(a), -math.sin(a), 0.0], [math.sin(a), math.cos(a), 0.0], [0.0, 0.0, 1.0]])
+    c4r = np.einsum("ia,jb,kc,ld,abcd->ijkl", r, r, r, r, c4)
+    out = np.zeros((6, 6))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for l in range(3):
+                    out[_VOIGT[i, j], _VOIGT[k, l]] = c4r[i, j, k, l]
+    return out
+
+
+def cij_cell_fields(c6_per_cell):
+    """21 CELL fields from an [nc, 6, 6] stiffness array."""
+    return [Field(FIELD_CELL, np.ascontiguousarray(c6_per_cell[:, i, j], dtype=np.float64)) for i, j in CIJ_PAIRS]
+
+
+def max_wave_speed(c6, rho):
+    return math.sqrt(float(np.max(np.diag(c6)[:3])) / rho)
+
+
+def cfl_dt(c, dx, dy, dz, fraction=0.9):
+    return fraction / (c * (1.0 / dx + 1.0 / dy + 1.0 / dz))
+
+
+# ---------------------------------------------------------------- signals and sensors
+
+def tone_burst(freq, cycles, dt, phase_deg=0.0):
+    """The case makers' Hann-windowed sine burst, normalised to max 1."""
+    from scipy.signal import windows
+
+    steps = int(math.ceil((cycles / freq) / dt))
+    t = np.arange(0, steps * dt, dt)[:steps]
+    wf = np.sin(t * freq * 2 * np.pi + math.radians(phase_deg)) * windows.hann(steps)
+    return t.astype(np.float64), (wf / wf.max()).astype(np.float64)
+
+
+def top_face_nodes(z, tol=1e-12):
+    return np.nonzero(np.abs(z - z.max()) <= tol * max(1.0, abs(z.max())))[0]
+
+
+def seeded_sensors(rng, candidates, count):
+    count = min(count, candidates.shape[0])
+    return np.sort(rng.choice(candidates, size=count, replace=False)).astype(np.int32)
+
+
+def cell0_dz(x, y, z, cells):
+    """|z(v0) - z(v6)| of cell 0 (Boss.hpp:530-536)."""
+    return abs(z[cells[0, 0]] - z[cells[0, 6]])
+
+
+# ---------------------------------------------------------------- the workloads
+
+def al_block(seed=1, nx=40, ny=40, nz=20, dx=1.0e-4, nsteps=500, n_sensors=16,
+             rho_type=FIELD_CELL):
+    """Isotropic aluminium block (rho 2700, lambda 51 GPa, mu 26 GPa), traction-free,
+    5 MHz 3-cycle burst on a 2 mm disc centred on the top face, dt = 0.9 CFL."""
+    rho, lam, mu = 2700.0, 51.0e9, 26.0e9
+    x, y, z, cells, _ = structured_mesh(nx, ny, nz, dx, dx, dx)
+    c6 = isotropic_c(lam, mu)
+    dt = cfl_dt(math.sqrt((lam + 2 * mu) / rho), dx, dx, dx)
+    t, v = tone_burst(5.0e6, 3, dt)
+    center = (nx * dx / 2, ny * dx / 2, nz * dx)
+    disc = Disc(2, center, 2.0e-3 * (nx / 40.0), cell0_dz(x, y, z, cells), t, v)
+    rng = np.random.default_rng(seed)
+    nc = cells.shape[0]
+    if rho_type == FIELD_NODE:
+        rho_f = Field(FIELD_NODE, np.full(x.shape[0], rho))
+    elif rho_type == FIELD_CONSTANT:
+        rho_f = Field(FIELD_CONSTANT, np.array([rho]))
+    else:
+        rho_f = Field(FIELD_CELL, np.full(nc, rho))
+    return Case(name="al_block_%dx%dx%d" % (nx, ny, nz), x=x, y=y, z=z, cells=cells, rho=rho_f,
+                cij=cij_cell_fields(np.broadcast_to(c6, (nc, 6, 6))), dt=dt, nsteps=nsteps,
+                force=[disc], sensors=seeded_sensors(rng, top_face_nodes(z), n_sensors), seed=seed)
+
+
+CFRP_PLY = dict(C11=135e9, C22=9.5e9, C33=9.5e9, C12=4.5e9, C13=4.5e9, C23=4.2e9, C44=2.6e9,
+                C55=5.0e9, C66=5.0e9)
+CFRP_LAYUP = [0, 45, 90, -45, 0, 45, 90, -45, -45, 90, 45, 0, -45, 90, 45, 0]
+
+
+def cfrp_ply_c():
+    p = CFRP_PLY
+    c = np.zeros((6, 6))
+    c[0, 0], c[1, 1], c[2, 2] = p["C11"], p["C22"], p["C33"]
+    c[0, 1] = c[1, 0] = p["C12"]
+    c[0, 2] = c[2, 0] = p["C13"]
+    c[1, 2] = c[2, 1] = p["C23"]
+    c[3, 3], c[4, 4], c[5, 5] = p["C44"], p["C55"], p["C66"]
+    return c
+
+
+def cfrp_plate(seed=2, nx=256, ny=256, cells_per_ply=4, dx=5.0e-5, nsteps=2000, n_holes=5,
+               n_sensors=16, renumber=True, xd_radius=2.0e-3):
+    """Orthotropic CFRP laminate, [0/45/90/-45]2s, 16 plies of ``cells_per_ply`` cells,
+    rho 1580, seeded through-holes of radius 1-3 mm (cells removed, orphan nodes
+    dropped), node ids seeded-permuted then RCM-renumbered, 2 MHz 5-cycle burst."""
+    rng = np.random.default_rng(seed)
+    nplies = len(CFRP_LAYUP)
+    nz = nplies * cells_per_ply
+    lx, ly = nx * dx, ny * dx
+    xc = (np.arange(nx) + 0.5) * dx
+    yc = (np.arange(ny) + 0.5) * dx
+    keep2d = np.ones((nx, ny), dtype=bool)
+    center = (lx / 2, ly / 2, nz * dx)
+    holes = []
+    scale = min(lx, ly) / 12.8e-3
+    for _ in range(n_holes):
+        for _attempt in range(200):
+            r = rng.uniform(1.0e-3, 3.0e-3) * scale
+            hx, hy = rng.uniform(r + dx, lx - r - dx), rng.uniform(r + dx, ly - r - dx)
+            if math.hypot(hx - center[0], hy - center[1]) < r + xd_radius * scale + 4 * dx:
+                continue
+            if any(math.hypot(hx - a, hy - b) < r + q + 4 * dx for a, b, q in holes):
+                continue
+            holes.append((hx, hy, r))
+            break
+    for hx, hy, r in holes:
+        keep2d &= (xc[:, None] - hx) ** 2 + (yc[None, :] - hy) ** 2 >= r * r
+    keep = np.repeat(keep2d[:, :, None], nz, axis=2)
+    x, y, z, cells, ijk = structured_mesh(nx, ny, nz, dx, dx, dx, keep=keep)
+    ply_c = np.stack([rotate_c_about_z(cfrp_ply_c(), a) for a in CFRP_LAYUP])
+    c_cells = ply_c[ijk[:, 2] // cells_per_ply]
+    rho = 1580.0
+    if renumber:
+        perm = rng.permutation(x.shape[0])
+        x, y, z, cells = renumber_nodes(x, y, z, cells, perm)
+        x, y, z, cells = renumber_nodes(x, y, z, cells, rcm_order(x.shape[0], cells))
+    dt = cfl_dt(max_wave_speed(cfrp_ply_c(), rho), dx, dx, dx)
+    t, v = tone_burst(2.0e6, 5, dt)
+    disc = Disc(2, center, xd_radius * scale, dx, t, v)
+    return Case(name="cfrp_plate_%dx%dx%d" % (nx, ny, nz), x=x, y=y, z=z, cells=cells,
+                rho=Field(FIELD_CELL, np.full(cells.shape[0], rho)), cij=cij_cell_fields(c_cells),
+                dt=dt, nsteps=nsteps, force=[disc],
+                sensors=seeded_sensors(rng, top_face_nodes(z), n_sensors), seed=seed)
+
+
+def water_column_block(seed=3, nx=12, ny=10, nz=6, dx=1.0e-4, nsteps=40):
+    """HostUTModel form: isotropic block driven by an XX/YY/ZZ stress (moment) disc on
+    the top face, as WaterColumnModel applies it (WaterColumnModel.cpp:127-180)."""
+    case = al_block(seed=seed, nx=nx, ny=ny, nz=nz, dx=dx, nsteps=nsteps, n_sensors=8)
+    t, v = case.force[0].signal_t, case.force[0].signal_v
+    center = (nx * dx / 2, ny * dx / 2, nz * dx)
+    r = 0.3 * min(nx, ny) * dx
+    ds = case.force[0].ds
+    case.moment = [Disc(0, center, r, ds, t, v), Disc(1, center, r, ds, t, 0.5 * v),
+                   Disc(2, center, r, ds, t, -0.25 * v)]
+    case.force = []
+    case.moment_form = 1
+    case.name = "water_column_%dx%dx%d" % (nx, ny, nz)
+    return case

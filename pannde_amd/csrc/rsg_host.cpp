@@ -1,0 +1,1046 @@
+// rsg_host.cpp -- host half of libpannde_rsg.so: the C ABI declared in
+// include/pannde_rsg.h.
+//
+// Set-up (rsg_create) reproduces, in FP64 and in the reference's operation order, what
+// HostRSGTDUT / HostUTModel compute before their first step:
+//   * node-to-cell links, HexMesh::buildLinks (HexMesh.hpp:300-313): slot = vertex ^ 6;
+//   * spacings, RightHex::cellPrep / nodePrep (RightHex.hpp:253-309), including the
+//     mirrored boundary bound and the logic_error on a non-positive spacing;
+//   * density at nodes and C_IJ at cells with HostField::atNode / atCell
+//     (HostField.hpp:121-154): the /8 rule that counts missing cells as zero;
+//   * dt/rho_n, the factor updateVsAtNode recomputes every step (HostRSGTDUT.hpp:343).
+// It then reorders nodes and cells for locality (z, y, x lexicographic on quantised
+// coordinates, so a wavefront's gathers hit a few contiguous runs) and uploads
+// structure-of-arrays buffers.  Source channels become per-entity term lists in the
+// reference's summation order (evalXds: connected-cell slot, then transducer,
+// HostRSGTDUT.hpp:387-402; evalMoments: vertex, then excitation, HostUTModel.hpp:296-311).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/pannde_rsg.h"
+#include "rsg_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(RSG_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+// HostUnivariate (HostUnivariate.hpp:68-142): keys ascend as in a std::map, a repeated
+// key keeps the value emplaced first; evalAt holds the last value from the last key on
+// and interpolates (Dv/Dt)*(t-t_prev)+v_prev below it.
+struct Signal {
+  std::vector<double> t, v;
+  void build(int32_t n, const double* ts, const double* vs) {
+    std::vector<int32_t> idx(n);
+    for (int32_t i = 0; i < n; i++) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return ts[a] < ts[b]; });
+    t.clear();
+    v.clear();
+    for (int32_t i : idx) {
+      if (!t.empty() && !(t.back() < ts[i])) continue;  // equal key: first one stays
+      t.push_back(ts[i]);
+      v.push_back(vs[i]);
+    }
+  }
+  bool eval(double at, double* out) const {
+    const size_t m = t.size();
+    if (!(at < t[m - 1])) {
+      *out = v[m - 1];
+      return true;
+    }
+    size_t i = (size_t)(std::upper_bound(t.begin(), t.end(), at) - t.begin());
+    if (i == 0) return false;  // below the first key: undefined in the reference
+    const double Dt = t[i] - t[i - 1];
+    const double Dv = v[i] - v[i - 1];
+    *out = (Dv / Dt) * (at - t[i - 1]) + v[i - 1];
+    return true;
+  }
+};
+
+struct Channel {
+  int32_t kind, comp;
+  std::vector<int32_t> support;
+  bool has_signal = false;
+  Signal sig;
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return RSG_OK;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      n = 0;
+      return fail(RSG_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    return RSG_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+constexpr int kBlockSteps = 256;   // steps whose amplitudes are staged per upload
+constexpr int kTraceCapSteps = 4096;
+
+}  // namespace
+
+struct rsg_instance {
+  int32_t precision = RSG_FP64;
+  int32_t nn = 0, nc = 0;
+  int32_t moment_form = 0;
+  double dt = 0, time = 0;
+  int64_t steps = 0;
+  int block_v = 256, block_s = 256;
+
+  // permutations: internal index -> caller index, and the inverse
+  std::vector<int32_t> node_of, cell_of, node_at, cell_at;
+  // caller-numbered connectivity (kept for source term lists)
+  std::vector<int32_t> cells;       // [nc][8]
+  std::vector<int32_t> node_cells;  // [nn][8]
+  // FP64 host copies in internal order of the values that get NaN-boxed
+  std::vector<double> idt_h, dcx_h;
+
+  // device state and coefficients (T = double or float)
+  void* V[3] = {nullptr, nullptr, nullptr};
+  void* S[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void* idt = nullptr;
+  void* dn[3] = {nullptr, nullptr, nullptr};
+  void* C = nullptr;
+  int64_t ldc = 0;
+  void* dc[3] = {nullptr, nullptr, nullptr};
+  int32_t* ncell = nullptr;
+  int32_t* cnode = nullptr;
+  double* node_w = nullptr;
+  double* cell_w = nullptr;
+
+  // source / sensor tables
+  std::vector<Channel> channels;
+  std::vector<int32_t> sensors;
+  bool tables_dirty = true;
+  int32_t n_sp_node = 0, n_sp_cell = 0;
+  DevBuf<char> sp_node_val;   // T[nsp]
+  DevBuf<int32_t> sp_node_off;
+  DevBuf<int32_t> sp_node_sens;
+  DevBuf<int32_t> node_terms;
+  DevBuf<char> sp_cell_val;   // T[nsp]
+  DevBuf<int32_t> sp_cell_off;
+  DevBuf<int32_t> cell_terms;
+  DevBuf<double> amp[2];
+  double* stage[2] = {nullptr, nullptr};  // pinned host staging
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  bool stage_busy[2] = {false, false};
+  int cur = 0;
+  DevBuf<double> trace;
+  int64_t trace_dev_steps = 0;       // rows currently held on the device
+  std::vector<double> trace_host;    // rows flushed to the host
+  DevBuf<double> partial;
+
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  double t_vel_ms = 0, t_str_ms = 0;
+  int64_t t_launches = 0;
+
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocations;
+  int64_t dev_bytes = 0;
+
+  size_t elt() const { return precision == RSG_FP64 ? 8 : 4; }
+
+  int dalloc(void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      return fail(RSG_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    allocations.push_back(*p);
+    dev_bytes += (int64_t)bytes;
+    return RSG_OK;
+  }
+
+  ~rsg_instance() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocations) (void)hipFree(p);
+    sp_node_val.release();
+    sp_node_off.release();
+    sp_node_sens.release();
+    node_terms.release();
+    sp_cell_val.release();
+    sp_cell_off.release();
+    cell_terms.release();
+    amp[0].release();
+    amp[1].release();
+    trace.release();
+    partial.release();
+    for (int i = 0; i < 2; i++) {
+      if (stage[i]) (void)hipHostFree(stage[i]);
+      if (stage_ev[i]) (void)hipEventDestroy(stage_ev[i]);
+    }
+    for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int field_len(const rsg_field_desc& f, int32_t nn, int32_t nc) {
+  return f.type == RSG_FIELD_NODE ? nn : (f.type == RSG_FIELD_CELL ? nc : 1);
+}
+
+int check_field(const rsg_field_desc& f, const char* name) {
+  if (!f.data) return fail(RSG_ERR_ARG, "field %s: null data", name);
+  if (f.type < RSG_FIELD_NODE || f.type > RSG_FIELD_CONSTANT)
+    return fail(RSG_ERR_ARG, "field %s: type %d is not NODE/CELL/CONSTANT", name, f.type);
+  return RSG_OK;
+}
+
+// HostField::atNode (HostField.hpp:142-154).
+double field_at_node(const rsg_field_desc& f, const int32_t* ncells, int32_t n) {
+  if (f.type == RSG_FIELD_NODE) return f.data[n];
+  double v = 0.0;
+  for (int k = 0; k < 8; k++) {
+    const int32_t c = ncells[(size_t)n * 8 + k];
+    v += (-1 == c) ? 0. : (f.type == RSG_FIELD_CONSTANT ? f.data[0] : f.data[c]);
+  }
+  return v / 8.0;
+}
+
+// HostField::atCell (HostField.hpp:121-133).
+double field_at_cell(const rsg_field_desc& f, const int32_t* cells, int32_t c) {
+  if (f.type == RSG_FIELD_NODE) {
+    double v = 0.0;
+    for (int k = 0; k < 8; k++) v += f.data[cells[(size_t)c * 8 + k]];
+    return v / 8.0;
+  }
+  return f.type == RSG_FIELD_CONSTANT ? f.data[0] : f.data[c];
+}
+
+// Locality order: sort by (z, y, x) of the point quantised to the finest spacing, so
+// x-runs are contiguous and each wavefront's gathers touch a few short runs.
+std::vector<int32_t> locality_order(int32_t n, const std::vector<double>& px,
+                                    const std::vector<double>& py, const std::vector<double>& pz,
+                                    const double h[3], const double lo[3]) {
+  std::vector<std::pair<uint64_t, int32_t>> key((size_t)n);
+  for (int32_t i = 0; i < n; i++) {
+    const double p[3] = {px[i], py[i], pz[i]};
+    uint64_t q[3];
+    for (int d = 0; d < 3; d++) {
+      double s = std::floor((p[d] - lo[d]) / h[d] + 0.5);
+      if (!(s >= 0)) s = 0;
+      if (s > 2097151.0) s = 2097151.0;
+      q[d] = (uint64_t)s;
+    }
+    key[i] = {(q[2] << 42) | (q[1] << 21) | q[0], i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> order((size_t)n);
+  for (int32_t i = 0; i < n; i++) order[i] = key[i].second;
+  return order;
+}
+
+template <typename T>
+int upload(rsg_instance* h, void** dst, const std::vector<T>& src) {
+  int rc = h->dalloc(dst, src.size() * sizeof(T));
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return RSG_OK;
+}
+
+// Converts FP64 values to the handle's precision and uploads them.
+int upload_real(rsg_instance* h, void** dst, const std::vector<double>& src) {
+  if (h->precision == RSG_FP64) return upload(h, dst, src);
+  std::vector<float> f(src.size());
+  for (size_t i = 0; i < src.size(); i++) f[i] = (float)src[i];
+  return upload(h, dst, f);
+}
+
+int copy_real_to(rsg_instance* h, void* dst, const std::vector<double>& src) {
+  if (h->precision == RSG_FP64) {
+    HIP_TRY(hipMemcpy(dst, src.data(), src.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(src.size());
+    for (size_t i = 0; i < src.size(); i++) f[i] = (float)src[i];
+    HIP_TRY(hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  }
+  return RSG_OK;
+}
+
+// Builds the special-node / special-cell tables from the channels and sensors, and
+// NaN-boxes the idt / dcx entries of the special entities on the device.
+int build_tables(rsg_instance* h) {
+  const int32_t nn = h->nn, nc = h->nc;
+  // force channels covering each cell (caller numbering), in channel order
+  std::vector<std::vector<int32_t>> cell_ch, node_ch;
+  bool any_force = false, any_moment = false;
+  for (size_t k = 0; k < h->channels.size(); k++) {
+    const Channel& ch = h->channels[k];
+    if (ch.kind == RSG_FORCE) {
+      if (!any_force) cell_ch.assign(nc, {});
+      any_force = true;
+      for (int32_t c : ch.support) cell_ch[c].push_back((int32_t)k);
+    } else {
+      if (!any_moment) node_ch.assign(nn, {});
+      any_moment = true;
+      for (int32_t n : ch.support) node_ch[n].push_back((int32_t)k);
+    }
+  }
+  // special nodes: touched by a force channel, or a sensor (internal order)
+  std::vector<int32_t> sens_slot(nn, -1);
+  for (size_t s = 0; s < h->sensors.size(); s++) sens_slot[h->sensors[s]] = (int32_t)s;
+  std::vector<char> is_sp(nn, 0);
+  if (any_force)
+    for (int32_t c = 0; c < nc; c++)
+      if (!cell_ch[c].empty())
+        for (int v = 0; v < 8; v++) is_sp[h->cells[(size_t)c * 8 + v]] = 1;
+  for (int32_t s : h->sensors) is_sp[s] = 1;
+
+  std::vector<double> sp_idt;
+  std::vector<int32_t> sp_off, sp_sens, terms;
+  std::vector<double> idt = h->idt_h;
+  for (int32_t i = 0; i < nn; i++) {
+    const int32_t n = h->node_of[i];
+    if (!is_sp[n]) continue;
+    const int32_t sp = (int32_t)sp_idt.size();
+    if (h->precision == RSG_FP32 && sp > rsg::kMaxSpecialF32)
+      return fail(RSG_ERR_ARG, "too many source/sensor nodes for the FP32 form");
+    sp_idt.push_back(idt[i]);
+    sp_sens.push_back(sens_slot[n]);
+    for (int d = 0; d < 3; d++) {
+      sp_off.push_back((int32_t)terms.size());
+      if (!any_force) continue;
+      for (int kb = 0; kb < 8; kb++) {  // evalXds: slot, then transducer
+        const int32_t c = h->node_cells[(size_t)n * 8 + kb];
+        if (c == -1) continue;
+        for (int32_t k : cell_ch[c])
+          if (h->channels[k].comp == d) terms.push_back(k);
+      }
+    }
+    sp_off.push_back((int32_t)terms.size());
+    idt[i] = h->precision == RSG_FP64 ? rsg::box_f64(sp) : (double)rsg::box_f32(sp);
+  }
+  // special cells: a vertex lies in a moment channel's support
+  std::vector<double> sp_dcx;
+  std::vector<int32_t> csp_off, cterms;
+  std::vector<double> dcx = h->dcx_h;
+  if (any_moment) {
+    for (int32_t i = 0; i < nc; i++) {
+      const int32_t c = h->cell_of[i];
+      bool touched = false;
+      for (int v = 0; v < 8 && !touched; v++) touched = !node_ch[h->cells[(size_t)c * 8 + v]].empty();
+      if (!touched) continue;
+      const int32_t sp = (int32_t)sp_dcx.size();
+      if (h->precision == RSG_FP32 && sp > rsg::kMaxSpecialF32)
+        return fail(RSG_ERR_ARG, "too many source cells for the FP32 form");
+      sp_dcx.push_back(dcx[i]);
+      for (int ks = 0; ks < 6; ks++) {
+        csp_off.push_back((int32_t)cterms.size());
+        for (int kb = 0; kb < 8; kb++) {  // evalMoments: vertex, then excitation
+          const int32_t m = h->cells[(size_t)c * 8 + kb];
+          for (int32_t k : node_ch[m])
+            if (h->channels[k].comp == ks) cterms.push_back(k);
+        }
+      }
+      csp_off.push_back((int32_t)cterms.size());
+      dcx[i] = h->precision == RSG_FP64 ? rsg::box_f64(sp) : (double)rsg::box_f32(sp);
+    }
+  }
+  h->n_sp_node = (int32_t)sp_idt.size();
+  h->n_sp_cell = (int32_t)sp_dcx.size();
+
+  auto put_real = [&](DevBuf<char>& buf, const std::vector<double>& v) -> int {
+    int rc = buf.alloc(std::max<size_t>(1, v.size()) * h->elt());
+    if (rc) return rc;
+    if (v.empty()) return RSG_OK;
+    if (h->precision == RSG_FP64) {
+      HIP_TRY(hipMemcpy(buf.p, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> f(v.begin(), v.end());
+      HIP_TRY(hipMemcpy(buf.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    }
+    return RSG_OK;
+  };
+  auto put_int = [&](DevBuf<int32_t>& buf, const std::vector<int32_t>& v) -> int {
+    int rc = buf.alloc(std::max<size_t>(1, v.size()));
+    if (rc) return rc;
+    if (!v.empty()) HIP_TRY(hipMemcpy(buf.p, v.data(), v.size() * 4, hipMemcpyHostToDevice));
+    return RSG_OK;
+  };
+  int rc;
+  if ((rc = put_real(h->sp_node_val, sp_idt))) return rc;
+  if ((rc = put_int(h->sp_node_off, sp_off))) return rc;
+  if ((rc = put_int(h->sp_node_sens, sp_sens))) return rc;
+  if ((rc = put_int(h->node_terms, terms))) return rc;
+  if ((rc = put_real(h->sp_cell_val, sp_dcx))) return rc;
+  if ((rc = put_int(h->sp_cell_off, csp_off))) return rc;
+  if ((rc = put_int(h->cell_terms, cterms))) return rc;
+  if ((rc = copy_real_to(h, h->idt, idt))) return rc;
+  if ((rc = copy_real_to(h, h->dc[0], dcx))) return rc;
+  // amplitude staging for kBlockSteps steps
+  const size_t nch = std::max<size_t>(1, h->channels.size());
+  for (int b = 0; b < 2; b++) {
+    if (h->stage_ev[b]) HIP_TRY(hipEventSynchronize(h->stage_ev[b]));
+    h->stage_busy[b] = false;
+    if ((rc = h->amp[b].alloc(nch * kBlockSteps))) return rc;
+    if (h->stage[b]) (void)hipHostFree(h->stage[b]);
+    h->stage[b] = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&h->stage[b], nch * kBlockSteps * sizeof(double), hipHostMallocDefault));
+    if (!h->stage_ev[b]) HIP_TRY(hipEventCreateWithFlags(&h->stage_ev[b], hipEventDisableTiming));
+  }
+  h->tables_dirty = false;
+  return RSG_OK;
+}
+
+int flush_traces(rsg_instance* h) {
+  const size_t row = h->sensors.size() * 3;
+  if (h->trace_dev_steps == 0 || row == 0) return RSG_OK;
+  const size_t old = h->trace_host.size();
+  h->trace_host.resize(old + (size_t)h->trace_dev_steps * row);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(h->trace_host.data() + old, h->trace.p,
+                    (size_t)h->trace_dev_steps * row * sizeof(double), hipMemcpyDeviceToHost));
+  h->trace_dev_steps = 0;
+  return RSG_OK;
+}
+
+template <typename T>
+void fill_args(rsg_instance* h, rsg::VelocityArgs<T>& va, rsg::StressArgs<T>& sa) {
+  va.V0 = (T*)h->V[0];
+  va.V1 = (T*)h->V[1];
+  va.V2 = (T*)h->V[2];
+  va.S0 = (const T*)h->S[0];
+  va.S1 = (const T*)h->S[1];
+  va.S2 = (const T*)h->S[2];
+  va.S3 = (const T*)h->S[3];
+  va.S4 = (const T*)h->S[4];
+  va.S5 = (const T*)h->S[5];
+  va.idt = (const T*)h->idt;
+  va.dnx = (const T*)h->dn[0];
+  va.dny = (const T*)h->dn[1];
+  va.dnz = (const T*)h->dn[2];
+  va.ncell = h->ncell;
+  va.n = h->nn;
+  va.sp_idt = (const T*)h->sp_node_val.p;
+  va.sp_off = h->sp_node_off.p;
+  va.sp_sens = h->sp_node_sens.p;
+  va.term = h->node_terms.p;
+  sa.S0 = (T*)h->S[0];
+  sa.S1 = (T*)h->S[1];
+  sa.S2 = (T*)h->S[2];
+  sa.S3 = (T*)h->S[3];
+  sa.S4 = (T*)h->S[4];
+  sa.S5 = (T*)h->S[5];
+  sa.V0 = (const T*)h->V[0];
+  sa.V1 = (const T*)h->V[1];
+  sa.V2 = (const T*)h->V[2];
+  sa.C = (const T*)h->C;
+  sa.ldc = h->ldc;
+  sa.dcx = (const T*)h->dc[0];
+  sa.dcy = (const T*)h->dc[1];
+  sa.dcz = (const T*)h->dc[2];
+  sa.cnode = h->cnode;
+  sa.n = h->nc;
+  sa.dt = (T)h->dt;
+  sa.sp_dcx = (const T*)h->sp_cell_val.p;
+  sa.sp_off = h->sp_cell_off.p;
+  sa.term = h->cell_terms.p;
+}
+
+int ensure_events(rsg_instance* h, size_t n) {
+  while (h->tev.size() < n) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    h->tev.push_back(e);
+  }
+  return RSG_OK;
+}
+
+template <typename T>
+int run_block(rsg_instance* h, int kb, const double* amp_dev) {
+  rsg::VelocityArgs<T> va;
+  rsg::StressArgs<T> sa;
+  fill_args<T>(h, va, sa);
+  const size_t nch = std::max<size_t>(1, h->channels.size());
+  const size_t row = h->sensors.size() * 3;
+  if (h->timing) {
+    int rc = ensure_events(h, (size_t)2 * kb + 1);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(h->tev[0], h->stream));
+  }
+  for (int k = 0; k < kb; k++) {
+    va.amp = amp_dev + (size_t)k * nch;
+    sa.amp = va.amp;
+    va.trace = h->trace.p ? h->trace.p + (size_t)(h->trace_dev_steps + k) * row : nullptr;
+    hipError_t e;
+    if (sizeof(T) == 8)
+      e = rsg::launch_velocity_f64(*(rsg::VelocityArgs<double>*)&va, h->block_v, h->stream);
+    else
+      e = rsg::launch_velocity_f32(*(rsg::VelocityArgs<float>*)&va, h->block_v, h->stream);
+    if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "velocity launch: %s", hipGetErrorString(e));
+    if (h->timing) HIP_TRY(hipEventRecord(h->tev[2 * k + 1], h->stream));
+    if (sizeof(T) == 8)
+      e = rsg::launch_stress_f64(*(rsg::StressArgs<double>*)&sa, h->block_s, h->stream);
+    else
+      e = rsg::launch_stress_f32(*(rsg::StressArgs<float>*)&sa, h->block_s, h->stream);
+    if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "stress launch: %s", hipGetErrorString(e));
+    if (h->timing) HIP_TRY(hipEventRecord(h->tev[2 * k + 2], h->stream));
+  }
+  if (h->timing) {
+    HIP_TRY(hipEventSynchronize(h->tev[2 * kb]));
+    for (int k = 0; k < kb; k++) {
+      float a = 0, b = 0;
+      HIP_TRY(hipEventElapsedTime(&a, h->tev[2 * k], h->tev[2 * k + 1]));
+      HIP_TRY(hipEventElapsedTime(&b, h->tev[2 * k + 1], h->tev[2 * k + 2]));
+      h->t_vel_ms += a;
+      h->t_str_ms += b;
+    }
+    h->t_launches += kb;
+  }
+  h->trace_dev_steps += row ? kb : 0;
+  return RSG_OK;
+}
+
+int check_mesh(const rsg_mesh_desc* m) {
+  if (!m) return fail(RSG_ERR_ARG, "mesh is null");
+  if (m->n_nodes <= 0 || m->n_cells <= 0)
+    return fail(RSG_ERR_ARG, "mesh needs nodes and cells (got %d, %d)", m->n_nodes, m->n_cells);
+  if (!m->x || !m->y || !m->z || !m->cell_nodes) return fail(RSG_ERR_ARG, "mesh array is null");
+  for (int64_t i = 0; i < (int64_t)m->n_cells * 8; i++) {
+    const int32_t v = m->cell_nodes[i];
+    if (v < 0 || v >= m->n_nodes)
+      return fail(RSG_ERR_ARG, "invalid node id: Node count: %d, provided index: %d", m->n_nodes, v);
+  }
+  return RSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rsg_last_error(char* buf, size_t len) {
+  if (buf && len) {
+    size_t k = std::min(len - 1, g_last_error.size());
+    memcpy(buf, g_last_error.data(), k);
+    buf[k] = 0;
+  }
+  return (int)g_last_error.size();
+}
+
+int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t precision,
+               rsg_handle* out) {
+  if (!out) return fail(RSG_ERR_ARG, "out is null");
+  *out = nullptr;
+  int rc = check_mesh(mesh);
+  if (rc) return rc;
+  if (!model) return fail(RSG_ERR_ARG, "model is null");
+  if (precision != RSG_FP64 && precision != RSG_FP32)
+    return fail(RSG_ERR_ARG, "precision %d is neither RSG_FP64 nor RSG_FP32", precision);
+  if ((rc = check_field(model->density, "density"))) return rc;
+  for (int k = 0; k < 21; k++)
+    if ((rc = check_field(model->cij[k], "C_IJ"))) return rc;
+  if (!std::isfinite(model->dt)) return fail(RSG_ERR_ARG, "dt is not finite");
+
+  std::unique_ptr<rsg_instance> h(new (std::nothrow) rsg_instance());
+  if (!h) return fail(RSG_ERR_NOMEM, "host allocation failed");
+  const int32_t nn = mesh->n_nodes, nc = mesh->n_cells;
+  h->precision = precision;
+  h->nn = nn;
+  h->nc = nc;
+  h->dt = model->dt;
+  h->moment_form = model->moment_form ? 1 : 0;
+  if (const char* e = getenv("RSG_BLOCK_VELOCITY")) h->block_v = atoi(e);
+  if (const char* e = getenv("RSG_BLOCK_STRESS")) h->block_s = atoi(e);
+
+  // links (HexMesh::buildLinks)
+  h->cells.assign(mesh->cell_nodes, mesh->cell_nodes + (size_t)nc * 8);
+  h->node_cells.assign((size_t)nn * 8, -1);
+  for (int32_t k = 0; k < nc; k++)
+    for (int v = 0; v < 8; v++) h->node_cells[(size_t)h->cells[(size_t)k * 8 + v] * 8 + (v ^ 6)] = k;
+
+  const double* X[3] = {mesh->x, mesh->y, mesh->z};
+  // RightHex::cellPrep
+  std::vector<double> dcell[3];
+  for (int d = 0; d < 3; d++) dcell[d].resize(nc);
+  for (int32_t c = 0; c < nc; c++) {
+    const int32_t a = h->cells[(size_t)c * 8], b = h->cells[(size_t)c * 8 + 6];
+    for (int d = 0; d < 3; d++) dcell[d][c] = X[d][b] - X[d][a];
+  }
+  // RightHex::nodePrep
+  std::vector<double> dnode[3];
+  for (int d = 0; d < 3; d++) dnode[d].resize(nn);
+  for (int32_t n = 0; n < nn; n++) {
+    double pn[3], lo[3], hi[3];
+    for (int d = 0; d < 3; d++) pn[d] = lo[d] = hi[d] = X[d][n];
+    for (int kb = 0; kb < 8; kb++) {
+      const int32_t c = h->node_cells[(size_t)n * 8 + kb];
+      if (c == -1) continue;
+      for (int kbb = 0; kbb < 8; kbb++) {
+        const int32_t m = h->cells[(size_t)c * 8 + kbb];
+        for (int d = 0; d < 3; d++) {
+          const double p = X[d][m];
+          lo[d] = (p < lo[d]) ? p : lo[d];
+          hi[d] = (hi[d] < p) ? p : hi[d];
+        }
+      }
+    }
+    for (int d = 0; d < 3; d++) {
+      if (lo[d] == pn[d]) lo[d] = 2. * pn[d] - hi[d];
+      if (hi[d] == pn[d]) hi[d] = 2. * pn[d] - lo[d];
+    }
+    static const char* kAxis[3] = {"DX", "DY", "DZ"};
+    for (int d = 0; d < 3; d++) {
+      dnode[d][n] = 0.5 * (hi[d] - lo[d]);
+      if (0 >= dnode[d][n]) return fail(RSG_ERR_MESH, "%s (node %d)", kAxis[d], n);
+    }
+  }
+
+  // locality order
+  double lo[3], hmin[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = *std::min_element(X[d], X[d] + nn);
+    hmin[d] = INFINITY;
+    for (int32_t c = 0; c < nc; c++) {
+      const double s = std::fabs(dcell[d][c]);
+      if (s > 0 && s < hmin[d]) hmin[d] = s;
+    }
+    if (!std::isfinite(hmin[d])) hmin[d] = 1.0;
+  }
+  {
+    std::vector<double> cx(nc), cy(nc), cz(nc);
+    for (int32_t c = 0; c < nc; c++) {
+      double s[3] = {0, 0, 0};
+      for (int v = 0; v < 8; v++)
+        for (int d = 0; d < 3; d++) s[d] += X[d][h->cells[(size_t)c * 8 + v]];
+      cx[c] = s[0] / 8;
+      cy[c] = s[1] / 8;
+      cz[c] = s[2] / 8;
+    }
+    double clo[3] = {lo[0] + 0.5 * hmin[0], lo[1] + 0.5 * hmin[1], lo[2] + 0.5 * hmin[2]};
+    h->cell_of = locality_order(nc, cx, cy, cz, hmin, clo);
+    std::vector<double> nx(X[0], X[0] + nn), ny(X[1], X[1] + nn), nz(X[2], X[2] + nn);
+    h->node_of = locality_order(nn, nx, ny, nz, hmin, lo);
+  }
+  h->cell_at.assign(nc, -1);
+  h->node_at.assign(nn, -1);
+  for (int32_t i = 0; i < nc; i++) h->cell_at[h->cell_of[i]] = i;
+  for (int32_t i = 0; i < nn; i++) h->node_at[h->node_of[i]] = i;
+
+  HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
+  // node arrays
+  {
+    std::vector<double> idt(nn), w(nn), dn[3];
+    for (int d = 0; d < 3; d++) dn[d].resize(nn);
+    std::vector<int32_t> conn((size_t)nn * 8);
+    for (int32_t i = 0; i < nn; i++) {
+      const int32_t n = h->node_of[i];
+      const double rho = field_at_node(model->density, h->node_cells.data(), n);
+      idt[i] = h->dt / (rho);
+      if (!(idt[i] == idt[i]) || std::isinf(idt[i]))
+        return fail(RSG_ERR_ARG, "dt/density is not finite at node %d (density %g)", n, rho);
+      w[i] = rho * 8.0 * dnode[0][n] * dnode[1][n] * dnode[2][n];
+      for (int d = 0; d < 3; d++)
+        dn[d][i] = precision == RSG_FP64 ? dnode[d][n] : 0.25 / dnode[d][n];
+      for (int k = 0; k < 8; k++) {
+        const int32_t c = h->node_cells[(size_t)n * 8 + k];
+        conn[(size_t)i * 8 + k] = c < 0 ? -1 : h->cell_at[c];
+      }
+    }
+    h->idt_h = idt;
+    if ((rc = upload_real(h.get(), &h->idt, idt))) return rc;
+    for (int d = 0; d < 3; d++)
+      if ((rc = upload_real(h.get(), &h->dn[d], dn[d]))) return rc;
+    void* p;
+    if ((rc = upload(h.get(), &p, conn))) return rc;
+    h->ncell = (int32_t*)p;
+    if ((rc = upload(h.get(), &p, w))) return rc;
+    h->node_w = (double*)p;
+  }
+  // cell arrays
+  {
+    std::vector<double> dc[3], w(nc);
+    for (int d = 0; d < 3; d++) dc[d].resize(nc);
+    std::vector<int32_t> conn((size_t)nc * 8);
+    h->ldc = ((int64_t)nc + 63) / 64 * 64;
+    std::vector<double> C((size_t)21 * h->ldc, 0.0);
+    for (int32_t i = 0; i < nc; i++) {
+      const int32_t c = h->cell_of[i];
+      for (int d = 0; d < 3; d++)
+        dc[d][i] = precision == RSG_FP64 ? dcell[d][c] : 0.25 / dcell[d][c];
+      w[i] = dcell[0][c] * dcell[1][c] * dcell[2][c];
+      for (int k = 0; k < 8; k++) conn[(size_t)i * 8 + k] = h->node_at[h->cells[(size_t)c * 8 + k]];
+      for (int j = 0; j < 21; j++) {
+        const double cij = field_at_cell(model->cij[j], h->cells.data(), c);
+        C[(size_t)j * h->ldc + i] = precision == RSG_FP64 ? cij : h->dt * cij;
+      }
+    }
+    h->dcx_h = dc[0];
+    for (int d = 0; d < 3; d++)
+      if ((rc = upload_real(h.get(), &h->dc[d], dc[d]))) return rc;
+    if ((rc = upload_real(h.get(), &h->C, C))) return rc;
+    void* p;
+    if ((rc = upload(h.get(), &p, conn))) return rc;
+    h->cnode = (int32_t*)p;
+    if ((rc = upload(h.get(), &p, w))) return rc;
+    h->cell_w = (double*)p;
+  }
+  // zero state (TransducerModel.cpp:150-159)
+  for (int d = 0; d < 3; d++) {
+    if ((rc = h->dalloc(&h->V[d], (size_t)nn * h->elt()))) return rc;
+    HIP_TRY(hipMemset(h->V[d], 0, (size_t)nn * h->elt()));
+  }
+  for (int s = 0; s < 6; s++) {
+    if ((rc = h->dalloc(&h->S[s], (size_t)nc * h->elt()))) return rc;
+    HIP_TRY(hipMemset(h->S[s], 0, (size_t)nc * h->elt()));
+  }
+  if ((rc = h->partial.alloc(2 * 1024))) return rc;
+  HIP_TRY(hipDeviceSynchronize());
+  *out = h.release();
+  return RSG_OK;
+}
+
+int rsg_set_channels(rsg_handle h, int32_t n, const rsg_channel_desc* ch) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (n < 0 || (n > 0 && !ch)) return fail(RSG_ERR_ARG, "bad channel list");
+  std::vector<Channel> list((size_t)n);
+  for (int32_t k = 0; k < n; k++) {
+    const rsg_channel_desc& d = ch[k];
+    Channel& c = list[k];
+    c.kind = d.kind;
+    c.comp = d.component;
+    if (d.kind == RSG_FORCE) {
+      if (d.component < 0 || d.component > 2) return fail(RSG_ERR_ARG, "channel %d: force direction %d", k, d.component);
+    } else if (d.kind == RSG_MOMENT) {
+      if (!h->moment_form)
+        return fail(RSG_ERR_ARG, "channel %d: stress sources need the HostUTModel form (moment_form=1)", k);
+      if (d.component < 0 || d.component > 5) return fail(RSG_ERR_ARG, "channel %d: Voigt index %d", k, d.component);
+    } else {
+      return fail(RSG_ERR_ARG, "channel %d: kind %d", k, d.kind);
+    }
+    const int32_t lim = d.kind == RSG_FORCE ? h->nc : h->nn;
+    if (d.n_support < 0 || (d.n_support > 0 && !d.support)) return fail(RSG_ERR_ARG, "channel %d: support", k);
+    c.support.assign(d.support, d.support + d.n_support);
+    std::sort(c.support.begin(), c.support.end());
+    c.support.erase(std::unique(c.support.begin(), c.support.end()), c.support.end());
+    for (int32_t id : c.support)
+      if (id < 0 || id >= lim) return fail(RSG_ERR_ARG, "channel %d: support id %d out of range", k, id);
+    if (d.n_signal > 0) {
+      if (!d.signal_t || !d.signal_v) return fail(RSG_ERR_ARG, "channel %d: signal arrays", k);
+      c.has_signal = true;
+      c.sig.build(d.n_signal, d.signal_t, d.signal_v);
+    }
+  }
+  if (h->stream) HIP_TRY(hipStreamSynchronize(h->stream));
+  h->channels.swap(list);
+  h->tables_dirty = true;
+  return build_tables(h);
+}
+
+int rsg_set_sensors(rsg_handle h, int32_t n, const int32_t* nodes) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (n < 0 || (n > 0 && !nodes)) return fail(RSG_ERR_ARG, "bad sensor list");
+  for (int32_t s = 0; s < n; s++)
+    if (nodes[s] < 0 || nodes[s] >= h->nn) return fail(RSG_ERR_ARG, "sensor node %d out of range", nodes[s]);
+  std::vector<char> seen(h->nn, 0);
+  for (int32_t s = 0; s < n; s++) {
+    if (seen[nodes[s]]) return fail(RSG_ERR_ARG, "sensor node %d listed twice", nodes[s]);
+    seen[nodes[s]] = 1;
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->sensors.assign(nodes, nodes + n);
+  h->trace_host.clear();
+  h->trace_dev_steps = 0;
+  int rc = h->trace.alloc((size_t)std::max(1, n) * 3 * kTraceCapSteps);
+  if (rc) return rc;
+  h->tables_dirty = true;
+  return build_tables(h);
+}
+
+int rsg_step(rsg_handle h, int32_t nsteps, const double* amplitudes, double* time_out) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (nsteps < 0) return fail(RSG_ERR_ARG, "nsteps < 0");
+  int rc;
+  if (h->tables_dirty && (rc = build_tables(h))) return rc;
+  const size_t nch = h->channels.size();
+  if (!amplitudes)
+    for (size_t k = 0; k < nch; k++)
+      if (!h->channels[k].has_signal)
+        return fail(RSG_ERR_ARG, "channel %zu has no signal table and no amplitudes were given", k);
+  const size_t row = h->sensors.size() * 3;
+  for (int32_t done = 0; done < nsteps;) {
+    const int kb = std::min(kBlockSteps, nsteps - done);
+    if (row && h->trace_dev_steps + kb > kTraceCapSteps && (rc = flush_traces(h))) return rc;
+    const int b = h->cur;
+    if (nch) {
+      if (h->stage_busy[b]) HIP_TRY(hipEventSynchronize(h->stage_ev[b]));
+      double* st = h->stage[b];
+      double t = h->time;
+      for (int k = 0; k < kb; k++) {
+        for (size_t c = 0; c < nch; c++) {
+          double a;
+          if (amplitudes) {
+            a = amplitudes[(size_t)(done + k) * nch + c];
+          } else {
+            double s;
+            if (!h->channels[c].sig.eval(t, &s))
+              return fail(RSG_ERR_ARG, "channel %zu: time %g precedes its first signal time", c, t);
+            a = 0.125 * (s * 1.0);  // HostNormalXDZ::evalAt inside its window, times 0.125
+          }
+          st[(size_t)k * nch + c] = a;
+        }
+        t += h->dt;
+      }
+      HIP_TRY(hipMemcpyAsync(h->amp[b].p, st, (size_t)kb * nch * sizeof(double),
+                             hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(hipEventRecord(h->stage_ev[b], h->stream));
+      h->stage_busy[b] = true;
+    }
+    rc = h->precision == RSG_FP64 ? run_block<double>(h, kb, h->amp[b].p)
+                                  : run_block<float>(h, kb, h->amp[b].p);
+    if (rc) return rc;
+    for (int k = 0; k < kb; k++) h->time += h->dt;  // HostRSGTDUT::solve: time += dt
+    h->steps += kb;
+    h->cur ^= 1;
+    done += kb;
+  }
+  if (time_out) *time_out = h->time;
+  return RSG_OK;
+}
+
+int rsg_get_state(rsg_handle h, double* V, double* S) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const size_t e = h->elt();
+  std::vector<char> buf((size_t)std::max(h->nn, h->nc) * e);
+  auto fetch = [&](void* src, int32_t n, const std::vector<int32_t>& of, double* dst) -> int {
+    HIP_TRY(hipMemcpy(buf.data(), src, (size_t)n * e, hipMemcpyDeviceToHost));
+    if (e == 8) {
+      const double* b = (const double*)buf.data();
+      for (int32_t i = 0; i < n; i++) dst[of[i]] = b[i];
+    } else {
+      const float* b = (const float*)buf.data();
+      for (int32_t i = 0; i < n; i++) dst[of[i]] = (double)b[i];
+    }
+    return RSG_OK;
+  };
+  int rc;
+  if (V)
+    for (int d = 0; d < 3; d++)
+      if ((rc = fetch(h->V[d], h->nn, h->node_of, V + (size_t)d * h->nn))) return rc;
+  if (S)
+    for (int s = 0; s < 6; s++)
+      if ((rc = fetch(h->S[s], h->nc, h->cell_of, S + (size_t)s * h->nc))) return rc;
+  return RSG_OK;
+}
+
+int rsg_set_state(rsg_handle h, const double* V, const double* S) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const size_t e = h->elt();
+  std::vector<char> buf((size_t)std::max(h->nn, h->nc) * e);
+  auto put = [&](void* dst, int32_t n, const std::vector<int32_t>& of, const double* src) -> int {
+    if (e == 8) {
+      double* b = (double*)buf.data();
+      for (int32_t i = 0; i < n; i++) b[i] = src[of[i]];
+    } else {
+      float* b = (float*)buf.data();
+      for (int32_t i = 0; i < n; i++) b[i] = (float)src[of[i]];
+    }
+    HIP_TRY(hipMemcpy(dst, buf.data(), (size_t)n * e, hipMemcpyHostToDevice));
+    return RSG_OK;
+  };
+  int rc;
+  if (V)
+    for (int d = 0; d < 3; d++)
+      if ((rc = put(h->V[d], h->nn, h->node_of, V + (size_t)d * h->nn))) return rc;
+  if (S)
+    for (int s = 0; s < 6; s++)
+      if ((rc = put(h->S[s], h->nc, h->cell_of, S + (size_t)s * h->nc))) return rc;
+  return RSG_OK;
+}
+
+int rsg_get_traces(rsg_handle h, int64_t first, int64_t n, double* out) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  int rc = flush_traces(h);
+  if (rc) return rc;
+  const size_t row = h->sensors.size() * 3;
+  const int64_t have = row ? (int64_t)(h->trace_host.size() / row) : 0;
+  if (first < 0 || n < 0 || first + n > have)
+    return fail(RSG_ERR_ARG, "trace steps [%lld, %lld) outside the %lld recorded", (long long)first,
+                (long long)(first + n), (long long)have);
+  if (n && !out) return fail(RSG_ERR_ARG, "out is null");
+  if (n) memcpy(out, h->trace_host.data() + (size_t)first * row, (size_t)n * row * sizeof(double));
+  return RSG_OK;
+}
+
+int rsg_synchronize(rsg_handle h) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return RSG_OK;
+}
+
+int rsg_get_time(rsg_handle h, double* time, int64_t* steps) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (time) *time = h->time;
+  if (steps) *steps = h->steps;
+  return RSG_OK;
+}
+
+int rsg_energy(rsg_handle h, double* kinetic, double* stress_norm) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  const int nb = 1024;
+  hipError_t e;
+  if (h->precision == RSG_FP64) {
+    rsg::EnergyArgs<double> a;
+    a.V0 = (const double*)h->V[0];
+    a.V1 = (const double*)h->V[1];
+    a.V2 = (const double*)h->V[2];
+    a.node_w = h->node_w;
+    a.nn = h->nn;
+    for (int s = 0; s < 6; s++) a.S[s] = (const double*)h->S[s];
+    a.cell_w = h->cell_w;
+    a.nc = h->nc;
+    a.partial = h->partial.p;
+    e = rsg::launch_energy_f64(a, nb, h->stream);
+  } else {
+    rsg::EnergyArgs<float> a;
+    a.V0 = (const float*)h->V[0];
+    a.V1 = (const float*)h->V[1];
+    a.V2 = (const float*)h->V[2];
+    a.node_w = h->node_w;
+    a.nn = h->nn;
+    for (int s = 0; s < 6; s++) a.S[s] = (const float*)h->S[s];
+    a.cell_w = h->cell_w;
+    a.nc = h->nc;
+    a.partial = h->partial.p;
+    e = rsg::launch_energy_f32(a, nb, h->stream);
+  }
+  if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "energy launch: %s", hipGetErrorString(e));
+  std::vector<double> p(2 * nb);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(p.data(), h->partial.p, p.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double k = 0, s = 0;
+  for (int i = 0; i < nb; i++) {
+    k += p[i];
+    s += p[nb + i];
+  }
+  if (kinetic) *kinetic = k;
+  if (stress_norm) *stress_norm = s;
+  return RSG_OK;
+}
+
+int rsg_set_timing(rsg_handle h, int32_t enabled) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  h->timing = enabled != 0;
+  return RSG_OK;
+}
+
+int rsg_get_kernel_times(rsg_handle h, double* vms, double* sms, int64_t* launches) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (vms) *vms = h->t_vel_ms;
+  if (sms) *sms = h->t_str_ms;
+  if (launches) *launches = h->t_launches;
+  h->t_vel_ms = h->t_str_ms = 0;
+  h->t_launches = 0;
+  return RSG_OK;
+}
+
+int rsg_device_bytes(rsg_handle h, int64_t* bytes) {
+  if (!h || !bytes) return fail(RSG_ERR_ARG, "null argument");
+  *bytes = h->dev_bytes + (int64_t)(h->sp_node_val.bytes() + h->sp_node_off.bytes() +
+                                    h->sp_node_sens.bytes() + h->node_terms.bytes() +
+                                    h->sp_cell_val.bytes() + h->sp_cell_off.bytes() +
+                                    h->cell_terms.bytes() + h->amp[0].bytes() + h->amp[1].bytes() +
+                                    h->trace.bytes() + h->partial.bytes());
+  return RSG_OK;
+}
+
+int rsg_destroy(rsg_handle h) {
+  delete h;
+  return RSG_OK;
+}
+
+int rsg_disc_support(const rsg_mesh_desc* mesh, int32_t at, const double center[3], double radius,
+                     double ds, int32_t* ids, int32_t* count) {
+  int rc = check_mesh(mesh);
+  if (rc) return rc;
+  if (!center || !ids || !count || (at != 0 && at != 1)) return fail(RSG_ERR_ARG, "bad argument");
+  auto sqr = [](double a) { return a * a; };
+  auto check = [&](const double p[3]) {  // HostNormalXDZ::checkAt
+    double res = 0.0;
+    for (int k = 0; k < 3; k++) res += sqr(p[k] - center[k]);
+    const double dist = std::sqrt(res);
+    const double distz = std::fabs(p[2] - center[2]);
+    return (distz <= 0.75 * ds) * (dist < radius) * (dist >= 0.);
+  };
+  int32_t m = 0;
+  if (at == 0) {
+    for (int32_t c = 0; c < mesh->n_cells; c++) {
+      double cc[3] = {0., 0., 0.};  // HostRSGTDUT::cellcenter
+      for (int kb = 0; kb < 8; kb++) {
+        const int32_t v = mesh->cell_nodes[(size_t)c * 8 + kb];
+        cc[0] += 0.125 * mesh->x[v];
+        cc[1] += 0.125 * mesh->y[v];
+        cc[2] += 0.125 * mesh->z[v];
+      }
+      if (check(cc)) ids[m++] = c;
+    }
+  } else {
+    for (int32_t n = 0; n < mesh->n_nodes; n++) {
+      const double p[3] = {mesh->x[n], mesh->y[n], mesh->z[n]};
+      if (check(p)) ids[m++] = n;
+    }
+  }
+  *count = m;
+  return RSG_OK;
+}
+
+int rsg_signal_eval(int32_t n, const double* t, const double* v, int32_t n_eval, const double* at,
+                    double* out) {
+  if (n <= 0 || !t || !v || n_eval < 0 || (n_eval && (!at || !out)))
+    return fail(RSG_ERR_ARG, "bad argument");
+  Signal s;
+  s.build(n, t, v);
+  for (int32_t i = 0; i < n_eval; i++)
+    if (!s.eval(at[i], &out[i])) return fail(RSG_ERR_ARG, "time %g precedes the first key", at[i]);
+  return RSG_OK;
+}
+
+}  // extern "C"

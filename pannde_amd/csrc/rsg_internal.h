@@ -1,0 +1,107 @@
+// rsg_internal.h -- device-side layout shared by the host code (rsg_host.cpp) and the
+// kernels (rsg_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <cstring>
+
+namespace rsg {
+
+// Kernel argument blocks.  All arrays are in the library's internal (locality) order.
+//
+// Source and sensor entities are flagged without any extra per-entity array: a node's
+// dt/rho (idt) or a cell's x-spacing (dcx) is replaced by a quiet NaN whose payload is
+// the entity's index into the small "special" tables, which hold the real value plus
+// the term list.  Ordinary entities therefore cost no extra HBM bytes.
+template <typename T>
+struct VelocityArgs {
+  T* V0;
+  T* V1;
+  T* V2;
+  const T* S0;  // Sxx
+  const T* S1;  // Syy
+  const T* S2;  // Szz
+  const T* S3;  // Syz
+  const T* S4;  // Sxz
+  const T* S5;  // Sxy
+  const T* idt;  // dt/rho_n, NaN-boxed for special nodes
+  const T* dnx;  // FP64: node spacing dxn; FP32: 0.25/dxn
+  const T* dny;
+  const T* dnz;
+  const int32_t* ncell;  // [n][8] connected cells by slot, -1 = none
+  int32_t n;
+  // special nodes (sources and/or sensors)
+  const T* sp_idt;         // [nsp] real dt/rho_n
+  const int32_t* sp_off;   // [nsp][4] term ranges for directions 0,1,2
+  const int32_t* sp_sens;  // [nsp] sensor slot or -1
+  const int32_t* term;     // channel id per term
+  const double* amp;       // [n_channels] amplitudes of this step
+  double* trace;           // [n_sensors][3] row of this step
+};
+
+template <typename T>
+struct StressArgs {
+  T* S0;
+  T* S1;
+  T* S2;
+  T* S3;
+  T* S4;
+  T* S5;
+  const T* V0;
+  const T* V1;
+  const T* V2;
+  const T* C;    // [21][ldc]: FP64 C_IJ; FP32 dt*C_IJ (FP64 product, rounded)
+  int64_t ldc;   // leading dimension of C (>= n)
+  const T* dcx;  // FP64: cell spacing dxc (NaN-boxed for special cells); FP32: 0.25/dxc
+  const T* dcy;
+  const T* dcz;
+  const int32_t* cnode;  // [n][8] vertices
+  int32_t n;
+  T dt;  // used by the FP64 form only
+  // special cells (stress sources)
+  const T* sp_dcx;        // [nsp] real x-spacing value
+  const int32_t* sp_off;  // [nsp][7] term ranges for Voigt components 0..5
+  const int32_t* term;
+  const double* amp;
+};
+
+// Energy reduction arguments (FP64 accumulation, one partial per block).
+template <typename T>
+struct EnergyArgs {
+  const T* V0;
+  const T* V1;
+  const T* V2;
+  const double* node_w;  // rho_n * 8*dxn*dyn*dzn (FP64)
+  int32_t nn;
+  const T* S[6];
+  const double* cell_w;  // dxc*dyc*dzc
+  int32_t nc;
+  double* partial;  // [2][nblocks]
+};
+
+// Launchers (rsg_kernels.hip).  block = threads per block (64..1024, multiple of 64).
+hipError_t launch_velocity_f64(const VelocityArgs<double>& a, int block, hipStream_t s);
+hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStream_t s);
+hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t s);
+hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t s);
+hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s);
+hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nblocks, hipStream_t s);
+
+// NaN boxing of a special-table index.
+inline double box_f64(int32_t idx) {
+  uint64_t b = 0x7FF8000000000000ull | (uint64_t)(uint32_t)idx;
+  double d;
+  std::memcpy(&d, &b, 8);
+  return d;
+}
+inline float box_f32(int32_t idx) {
+  uint32_t b = 0x7FC00000u | (uint32_t)idx;
+  float f;
+  std::memcpy(&f, &b, 4);
+  return f;
+}
+constexpr int32_t kMaxSpecialF32 = (1 << 22) - 1;
+
+}  // namespace rsg

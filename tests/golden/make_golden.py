@@ -1,0 +1,80 @@
+"""Writes the golden fixtures: inputs and outputs of the reference's own RSG solver.
+
+Each fixture holds a small job (mesh, material fields, sources, sensors, dt, steps) and
+what the reference HostRSGTDUT / HostUTModel computed for it (fields after N steps, the
+time solve() returned, the per-step sensor velocities).  The outputs come from
+oracle/_ref/ref_rsg, i.e. the reference headers compiled as they lie in the reference
+tree (``make -C oracle ref``), never from this project's code.
+
+Run from the repository root in a tree where the reference is present:
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from pannde_amd import cases  # noqa: E402
+from oracle import oracle_rsg as orc  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def fixture_cases():
+    out = []
+    c = cases.al_block(seed=11, nx=8, ny=7, nz=5, nsteps=25, n_sensors=5)
+    c.name = "al_block_cellrho"
+    out.append((c, 1))
+    c = cases.al_block(seed=12, nx=8, ny=7, nz=5, nsteps=25, n_sensors=5, rho_type=cases.FIELD_NODE)
+    c.name = "al_block_noderho"
+    out.append((c, 1))
+    c = cases.water_column_block(seed=13, nx=10, ny=9, nz=5, nsteps=25)
+    c.name = "water_column_moment"
+    out.append((c, 1))
+    c = cases.cfrp_plate(seed=14, nx=16, ny=14, cells_per_ply=1, dx=7.0e-4, nsteps=25, n_holes=2)
+    c.name = "cfrp_holes_rcm"
+    out.append((c, 3))  # the reference run under mpirun -n 3 (its MPI halo path)
+    c = cases.al_block(seed=15, nx=10, ny=8, nz=5, nsteps=25, n_sensors=6)
+    d = c.force[0]
+    c.force.append(cases.Disc(0, (d.center[0] * 0.8, d.center[1], d.center[2]), d.radius, d.ds, d.signal_t,
+                              -0.5 * d.signal_v))
+    c.force.append(cases.Disc(2, (d.center[0] * 1.1, d.center[1] * 0.9, d.center[2]), d.radius * 0.7, d.ds,
+                              d.signal_t, 0.3 * d.signal_v))
+    c.name = "three_transducers"
+    out.append((c, 1))
+    return out
+
+
+def save(case, ranks):
+    V, S, t, tr, _ = orc.run_reference(case, ranks=ranks)
+    arrays = dict(x=case.x, y=case.y, z=case.z, cells=case.cells, rho_type=case.rho.type, rho=case.rho.data,
+                  dt=case.dt, nsteps=case.nsteps, moment_form=case.moment_form, sensors=case.sensors,
+                  ref_V=V, ref_S=S, ref_time=t, ref_traces=tr, ref_ranks=ranks)
+    for k, f in enumerate(case.cij):
+        arrays["c%02d_type" % k] = f.type
+        arrays["c%02d" % k] = f.data
+    for kind, discs in (("force", case.force), ("moment", case.moment)):
+        arrays["n_" + kind] = len(discs)
+        for i, d in enumerate(discs):
+            p = "%s%d_" % (kind, i)
+            arrays[p + "comp"] = d.component
+            arrays[p + "center"] = np.asarray(d.center, dtype=np.float64)
+            arrays[p + "radius"] = d.radius
+            arrays[p + "ds"] = d.ds
+            arrays[p + "t"] = d.signal_t
+            arrays[p + "v"] = d.signal_v
+    path = os.path.join(OUT, case.name + ".npz")
+    np.savez_compressed(path, **arrays)
+    print(path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    if not orc.have_reference():
+        sys.exit("oracle/_ref/ref_rsg is missing: run make -C oracle ref where the reference is present")
+    for c, r in fixture_cases():
+        save(c, r)

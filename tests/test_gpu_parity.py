@@ -1,0 +1,172 @@
+"""Parity of the FP64 DeviceSolver (HIP, through the C ABI) with the reference.
+
+* every golden fixture: fields, time and sensor traces bit-identical to what the
+  reference solver computed (tests/golden, made by oracle/_ref/ref_rsg);
+* seeded jobs of every flavour: bit-identical to the oracle;
+* the Al-block workload (40x40x20 cells, 500 steps) in full: bit-identical to the oracle;
+* the CFRP-plate workload at full size through size-independent properties: the input
+  node numbering does not change one bit of the result, doubling the source doubles every
+  field exactly, and a zero source leaves the zero state untouched.
+The tolerance of the north-star (1e-10 relative max-norm) is asserted too, but the
+device reproduces the reference's operation order, so equality is exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden_names, load_golden
+from oracle import oracle_rsg as orc
+from pannde_amd import cases, rsg
+
+pytestmark = pytest.mark.gpu
+
+
+def relmax(a, b):
+    s = np.abs(b).max()
+    return np.abs(a - b).max() / (s if s > 0 else 1.0)
+
+
+def run_device(case, nsteps=None, precision=rsg.RSG_FP64, V0=None, S0=None):
+    s = rsg.DeviceSolver.from_case(case, precision)
+    try:
+        if V0 is not None or S0 is not None:
+            s.set_state(V0, S0)
+        t = s.solve(case.nsteps if nsteps is None else nsteps)
+        V, S = s.get_state()
+        return V, S, t, s.traces()
+    finally:
+        s.close()
+
+
+def assert_same(dev, ref):
+    V, S, t, tr = dev
+    V0, S0, t0, tr0 = ref
+    assert t == t0
+    for k in range(3):
+        assert relmax(V[k], V0[k]) <= 1e-10
+    for k in range(6):
+        assert relmax(S[k], S0[k]) <= 1e-10
+    assert np.array_equal(V, V0), "V differs: max %.3e" % np.abs(V - V0).max()
+    assert np.array_equal(S, S0), "S differs: max %.3e" % np.abs(S - S0).max()
+    assert np.array_equal(tr, tr0)
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_device_matches_reference_fixture(name):
+    case, ref = load_golden(name)
+    assert_same(run_device(case), (ref["V"], ref["S"], ref["time"], ref["traces"]))
+
+
+@pytest.mark.parametrize("block", [64, 256, 1024])
+def test_block_size_does_not_change_results(block, monkeypatch):
+    monkeypatch.setenv("RSG_BLOCK_VELOCITY", str(block))
+    monkeypatch.setenv("RSG_BLOCK_STRESS", str(block))
+    case, ref = load_golden("cfrp_holes_rcm")
+    assert_same(run_device(case), (ref["V"], ref["S"], ref["time"], ref["traces"]))
+
+
+@pytest.mark.parametrize("seed", [201, 202, 203, 204])
+def test_device_matches_oracle_on_seeded_jobs(seed):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = (int(v) for v in rng.integers(5, 14, size=3))
+    if seed % 2:
+        case = cases.water_column_block(seed=seed, nx=nx, ny=ny, nz=nz, nsteps=30)
+    else:
+        case = cases.al_block(seed=seed, nx=nx, ny=ny, nz=nz, nsteps=30, n_sensors=5,
+                              rho_type=int(rng.integers(0, 3)))
+    for f in case.cij:
+        f.data = f.data * (1 + 0.05 * rng.standard_normal(f.data.shape))
+    V0 = 1e-9 * rng.standard_normal((3, case.n_nodes))
+    S0 = 1e3 * rng.standard_normal((6, case.n_cells))
+    assert_same(run_device(case, V0=V0, S0=S0), orc.run_oracle(case, V0=V0, S0=S0))
+
+
+def test_al_block_workload_full():
+    """40x40x20 aluminium block, 5 MHz burst, 500 steps, 16 top-face sensors."""
+    case = cases.al_block(seed=1)
+    assert case.n_cells == 32000 and case.n_nodes == 35301
+    dev = run_device(case)
+    assert np.abs(dev[3]).max() > 0
+    assert_same(dev, orc.run_oracle(case))
+
+
+def test_split_solve_calls_equal_one_call():
+    case = cases.al_block(seed=3, nx=12, ny=10, nz=6, nsteps=300, n_sensors=4)
+    s = rsg.DeviceSolver.from_case(case)
+    for n in (1, 7, 255, 37):
+        s.solve(n)
+    V, S = s.get_state()
+    tr = s.traces()
+    s.close()
+    V1, S1, _, tr1 = run_device(case)
+    assert np.array_equal(V, V1) and np.array_equal(S, S1) and np.array_equal(tr, tr1)
+
+
+def test_caller_supplied_amplitudes_equal_library_signal():
+    case = cases.al_block(seed=4, nx=10, ny=9, nz=5, nsteps=60, n_sensors=3)
+    d = case.force[0]
+    s = rsg.DeviceSolver.from_case(case)
+    sup = rsg.disc_support((case.x, case.y, case.z, case.cells), 0, d.center, d.radius, d.ds)
+    s.set_channels([(rsg.RSG_FORCE, 2, sup, None, None)])
+    t = 0.0
+    amps = []
+    for _ in range(case.nsteps):
+        amps.append(0.125 * (orc.signal_eval(d.signal_t, d.signal_v, [t])[0] * 1.0))
+        t += case.dt
+    s.solve(case.nsteps, np.array(amps).reshape(-1, 1))
+    V, S = s.get_state()
+    s.close()
+    V0, S0, _, _ = orc.run_oracle(case)
+    assert np.array_equal(V, V0) and np.array_equal(S, S0)
+
+
+def test_state_round_trip_and_errors():
+    case = cases.water_column_block(nx=6, ny=5, nz=4, nsteps=5)
+    s = rsg.DeviceSolver.from_case(case)
+    rng = np.random.default_rng(9)
+    V0, S0 = rng.standard_normal((3, case.n_nodes)), rng.standard_normal((6, case.n_cells))
+    s.set_state(V0, S0)
+    V, S = s.get_state()
+    assert np.array_equal(V, V0) and np.array_equal(S, S0)
+    with pytest.raises(rsg.RSGError):
+        s.set_sensors([case.n_nodes + 1])
+    s.close()
+    plain = cases.al_block(nx=6, ny=5, nz=4, nsteps=5)
+    p = rsg.DeviceSolver.from_case(plain)
+    with pytest.raises(rsg.RSGError, match="moment_form"):
+        p.set_sources([], case.moment)
+    p.close()
+
+
+@pytest.fixture(scope="module")
+def plate_pair():
+    """The CFRP plate workload with the seeded RCM numbering and with the natural
+    case-maker numbering (same mesh, same holes)."""
+    a = cases.cfrp_plate(seed=2, nsteps=40)
+    b = cases.cfrp_plate(seed=2, nsteps=40, renumber=False)
+    return a, b
+
+
+def test_plate_numbering_invariance_full_size(plate_pair):
+    a, b = plate_pair
+    assert a.n_cells == b.n_cells and a.n_nodes == b.n_nodes and a.n_cells > 3_000_000
+    Va, Sa, ta, tra = run_device(a)
+    Vb, Sb, tb, trb = run_device(b)
+    key = lambda x, y, z: np.lexsort((np.round(z / 1e-6), np.round(y / 1e-6), np.round(x / 1e-6)))
+    ia, ib = key(a.x, a.y, a.z), key(b.x, b.y, b.z)
+    assert ta == tb and np.abs(Va).max() > 0
+    assert np.array_equal(Va[:, ia], Vb[:, ib])
+    assert np.array_equal(Sa, Sb)  # cells keep the case-maker order in both
+
+
+def test_plate_linearity_and_zero_source_full_size(plate_pair):
+    a, _ = plate_pair
+    V1, S1, _, tr1 = run_device(a, nsteps=25)
+    a.force[0].signal_v = 2.0 * a.force[0].signal_v
+    V2, S2, _, tr2 = run_device(a, nsteps=25)
+    a.force[0].signal_v = 0.5 * a.force[0].signal_v
+    assert np.array_equal(V2, 2 * V1) and np.array_equal(S2, 2 * S1) and np.array_equal(tr2, 2 * tr1)
+    a.force[0].signal_v = 0.0 * a.force[0].signal_v
+    V0, S0, _, _ = run_device(a, nsteps=5)
+    assert not np.any(V0) and not np.any(S0)
