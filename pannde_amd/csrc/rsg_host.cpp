@@ -292,12 +292,18 @@ int upload_real(rsg_instance* h, void** dst, const std::vector<double>& src) {
   return upload(h, dst, f);
 }
 
-int copy_real_to(rsg_instance* h, void* dst, const std::vector<double>& src) {
+// Uploads src in the handle's precision with the entries listed in boxes = {(index,
+// special id)} replaced by NaN boxes written directly in the target format.
+int copy_boxed(rsg_instance* h, void* dst, const std::vector<double>& src,
+               const std::vector<std::pair<int32_t, int32_t>>& boxes) {
   if (h->precision == RSG_FP64) {
-    HIP_TRY(hipMemcpy(dst, src.data(), src.size() * 8, hipMemcpyHostToDevice));
+    std::vector<double> d(src);
+    for (auto& b : boxes) d[b.first] = rsg::box_f64(b.second);
+    HIP_TRY(hipMemcpy(dst, d.data(), d.size() * 8, hipMemcpyHostToDevice));
   } else {
     std::vector<float> f(src.size());
     for (size_t i = 0; i < src.size(); i++) f[i] = (float)src[i];
+    for (auto& b : boxes) f[b.first] = rsg::box_f32(b.second);
     HIP_TRY(hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice));
   }
   return RSG_OK;
@@ -334,7 +340,8 @@ int build_tables(rsg_instance* h) {
 
   std::vector<double> sp_idt;
   std::vector<int32_t> sp_off, sp_sens, terms;
-  std::vector<double> idt = h->idt_h;
+  const std::vector<double>& idt = h->idt_h;
+  std::vector<std::pair<int32_t, int32_t>> node_boxes, cell_boxes;
   for (int32_t i = 0; i < nn; i++) {
     const int32_t n = h->node_of[i];
     if (!is_sp[n]) continue;
@@ -354,12 +361,12 @@ int build_tables(rsg_instance* h) {
       }
     }
     sp_off.push_back((int32_t)terms.size());
-    idt[i] = h->precision == RSG_FP64 ? rsg::box_f64(sp) : (double)rsg::box_f32(sp);
+    node_boxes.push_back({i, sp});
   }
   // special cells: a vertex lies in a moment channel's support
   std::vector<double> sp_dcx;
   std::vector<int32_t> csp_off, cterms;
-  std::vector<double> dcx = h->dcx_h;
+  const std::vector<double>& dcx = h->dcx_h;
   if (any_moment) {
     for (int32_t i = 0; i < nc; i++) {
       const int32_t c = h->cell_of[i];
@@ -379,7 +386,7 @@ int build_tables(rsg_instance* h) {
         }
       }
       csp_off.push_back((int32_t)cterms.size());
-      dcx[i] = h->precision == RSG_FP64 ? rsg::box_f64(sp) : (double)rsg::box_f32(sp);
+      cell_boxes.push_back({i, sp});
     }
   }
   h->n_sp_node = (int32_t)sp_idt.size();
@@ -411,8 +418,8 @@ int build_tables(rsg_instance* h) {
   if ((rc = put_real(h->sp_cell_val, sp_dcx))) return rc;
   if ((rc = put_int(h->sp_cell_off, csp_off))) return rc;
   if ((rc = put_int(h->cell_terms, cterms))) return rc;
-  if ((rc = copy_real_to(h, h->idt, idt))) return rc;
-  if ((rc = copy_real_to(h, h->dc[0], dcx))) return rc;
+  if ((rc = copy_boxed(h, h->idt, idt, node_boxes))) return rc;
+  if ((rc = copy_boxed(h, h->dc[0], dcx, cell_boxes))) return rc;
   // amplitude staging for kBlockSteps steps
   const size_t nch = std::max<size_t>(1, h->channels.size());
   for (int b = 0; b < 2; b++) {
