@@ -41,6 +41,9 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs + ["-o", tmp], verbose)
     os.replace(tmp, OUT)
+    # the HBM copy probe (bandwidth ceiling and counter calibration), a separate program
+    _run([HIPCC] + DEVICE_FLAGS[:5] + DEVICE_FLAGS[5:] + [os.path.join(CSRC, "hbm_copy_probe.hip"), "-o",
+                                                          os.path.join(HERE, "hbm_copy_probe")], verbose)
     return OUT
 
 

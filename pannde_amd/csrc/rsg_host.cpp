@@ -127,6 +127,10 @@ struct rsg_instance {
   double dt = 0, time = 0;
   int64_t steps = 0;
   int block_v = 256, block_s = 256;
+  int xcd = 1;                    // XCD-contiguous workgroup remap
+  int lds = 1;                    // stage gathers through LDS when the block sets fit
+  int c_blocked = 1;              // C as [n/64][21][64] instead of [21][n]
+  int tile[3] = {8, 8, 4};        // locality brick (x, y, z) in cells
 
   // permutations: internal index -> caller index, and the inverse
   std::vector<int32_t> node_of, cell_of, node_at, cell_at;
@@ -146,6 +150,15 @@ struct rsg_instance {
   void* dc[3] = {nullptr, nullptr, nullptr};
   int32_t* ncell = nullptr;
   int32_t* cnode = nullptr;
+  // LDS staging plans (null lconn: direct gathers)
+  int32_t* v_uoff = nullptr;
+  int32_t* v_ulist = nullptr;
+  uint16_t* v_lconn = nullptr;
+  int32_t v_maxu = 0;
+  int32_t* s_uoff = nullptr;
+  int32_t* s_ulist = nullptr;
+  uint16_t* s_lconn = nullptr;
+  int32_t s_maxu = 0;
   double* node_w = nullptr;
   double* cell_w = nullptr;
 
@@ -253,27 +266,77 @@ double field_at_cell(const rsg_field_desc& f, const int32_t* cells, int32_t c) {
   return f.type == RSG_FIELD_CONSTANT ? f.data[0] : f.data[c];
 }
 
-// Locality order: sort by (z, y, x) of the point quantised to the finest spacing, so
-// x-runs are contiguous and each wavefront's gathers touch a few short runs.
+// Locality order: the point quantised to the finest spacing, (qx, qy, qz), sorted by
+// bricks of tile[0] x tile[1] x tile[2] (brick z, brick y, brick x, then z, y, x inside
+// the brick).  tile = {huge, 1, 1} is plain (z, y, x) lexicographic order: x-runs are
+// contiguous, so each wavefront's gathers touch a few short runs.
 std::vector<int32_t> locality_order(int32_t n, const std::vector<double>& px,
                                     const std::vector<double>& py, const std::vector<double>& pz,
-                                    const double h[3], const double lo[3]) {
-  std::vector<std::pair<uint64_t, int32_t>> key((size_t)n);
+                                    const double h[3], const double lo[3], const int tile[3]) {
+  std::vector<uint32_t> q((size_t)n * 3);
+  uint64_t ext[3] = {1, 1, 1};
   for (int32_t i = 0; i < n; i++) {
     const double p[3] = {px[i], py[i], pz[i]};
-    uint64_t q[3];
     for (int d = 0; d < 3; d++) {
       double s = std::floor((p[d] - lo[d]) / h[d] + 0.5);
       if (!(s >= 0)) s = 0;
       if (s > 2097151.0) s = 2097151.0;
-      q[d] = (uint64_t)s;
+      q[(size_t)i * 3 + d] = (uint32_t)s;
+      ext[d] = std::max<uint64_t>(ext[d], (uint64_t)s + 1);
     }
-    key[i] = {(q[2] << 42) | (q[1] << 21) | q[0], i};
+  }
+  // brick (z, y, x), then (z, y, x) inside the brick; both fit 64 bits for extents < 2^21
+  const uint64_t t[3] = {std::min<uint64_t>((uint64_t)tile[0], ext[0]),
+                         std::min<uint64_t>((uint64_t)tile[1], ext[1]),
+                         std::min<uint64_t>((uint64_t)tile[2], ext[2])};
+  const uint64_t nb[3] = {(ext[0] + t[0] - 1) / t[0], (ext[1] + t[1] - 1) / t[1], (ext[2] + t[2] - 1) / t[2]};
+  std::vector<std::pair<std::pair<uint64_t, uint64_t>, int32_t>> key((size_t)n);
+  for (int32_t i = 0; i < n; i++) {
+    const uint64_t qx = q[(size_t)i * 3], qy = q[(size_t)i * 3 + 1], qz = q[(size_t)i * 3 + 2];
+    const uint64_t brick = ((qz / t[2]) * nb[1] + qy / t[1]) * nb[0] + qx / t[0];
+    const uint64_t inner = ((qz % t[2]) * t[1] + qy % t[1]) * t[0] + qx % t[0];
+    key[i] = {{brick, inner}, i};
   }
   std::sort(key.begin(), key.end());
   std::vector<int32_t> order((size_t)n);
   for (int32_t i = 0; i < n; i++) order[i] = key[i].second;
   return order;
+}
+
+// LDS staging plan for a gather kernel launched with `block` threads per workgroup:
+// workgroup b handles entities [b*block, (b+1)*block); its unique gather set (sorted
+// internal ids) is ulist[u_off[b]..u_off[b+1]) and each entity's 8 links become 16-bit
+// indices into that set (0xFFFF for a missing link).
+struct LdsPlan {
+  std::vector<int32_t> u_off, ulist;
+  std::vector<uint16_t> lconn;
+  int32_t max_u = 0;
+};
+
+bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int block, LdsPlan& p) {
+  const int32_t nb = (n + block - 1) / block;
+  p.u_off.assign((size_t)nb + 1, 0);
+  p.ulist.clear();
+  p.lconn.assign((size_t)n * 8, 0xFFFF);
+  p.max_u = 0;
+  std::vector<int32_t> ids;
+  ids.reserve((size_t)block * 8);
+  for (int32_t b = 0; b < nb; b++) {
+    const int32_t lo = b * block, hi = std::min(n, lo + block);
+    ids.clear();
+    for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++)
+      if (conn[i] >= 0) ids.push_back(conn[i]);
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    if (ids.size() >= 0xFFFF) return false;
+    for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++)
+      if (conn[i] >= 0)
+        p.lconn[i] = (uint16_t)(std::lower_bound(ids.begin(), ids.end(), conn[i]) - ids.begin());
+    p.ulist.insert(p.ulist.end(), ids.begin(), ids.end());
+    p.u_off[b + 1] = (int32_t)p.ulist.size();
+    p.max_u = std::max(p.max_u, (int32_t)ids.size());
+  }
+  return true;
 }
 
 template <typename T>
@@ -464,6 +527,11 @@ void fill_args(rsg_instance* h, rsg::VelocityArgs<T>& va, rsg::StressArgs<T>& sa
   va.dnz = (const T*)h->dn[2];
   va.ncell = h->ncell;
   va.n = h->nn;
+  va.xcd = h->xcd;
+  va.u_off = h->v_uoff;
+  va.ulist = h->v_ulist;
+  va.lconn = h->v_lconn;
+  va.max_u = h->v_maxu;
   va.sp_idt = (const T*)h->sp_node_val.p;
   va.sp_off = h->sp_node_off.p;
   va.sp_sens = h->sp_node_sens.p;
@@ -484,6 +552,11 @@ void fill_args(rsg_instance* h, rsg::VelocityArgs<T>& va, rsg::StressArgs<T>& sa
   sa.dcz = (const T*)h->dc[2];
   sa.cnode = h->cnode;
   sa.n = h->nc;
+  sa.xcd = h->xcd;
+  sa.u_off = h->s_uoff;
+  sa.ulist = h->s_ulist;
+  sa.lconn = h->s_lconn;
+  sa.max_u = h->s_maxu;
   sa.dt = (T)h->dt;
   sa.sp_dcx = (const T*)h->sp_cell_val.p;
   sa.sp_off = h->sp_cell_off.p;
@@ -594,6 +667,17 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
   h->moment_form = model->moment_form ? 1 : 0;
   if (const char* e = getenv("RSG_BLOCK_VELOCITY")) h->block_v = atoi(e);
   if (const char* e = getenv("RSG_BLOCK_STRESS")) h->block_s = atoi(e);
+  for (int blk : {h->block_v, h->block_s})
+    if (blk < 64 || blk > 512 || blk % 64 != 0)
+      return fail(RSG_ERR_ARG, "workgroup size %d: must be a multiple of 64 in [64, 512]", blk);
+  if (const char* e = getenv("RSG_XCD_REMAP")) h->xcd = atoi(e) != 0;
+  if (const char* e = getenv("RSG_LDS")) h->lds = atoi(e) != 0;
+  if (const char* e = getenv("RSG_C_BLOCKED")) h->c_blocked = atoi(e) != 0;
+  if (const char* e = getenv("RSG_TILE")) {
+    int t[3];
+    if (sscanf(e, "%d,%d,%d", &t[0], &t[1], &t[2]) == 3 && t[0] > 0 && t[1] > 0 && t[2] > 0)
+      for (int d = 0; d < 3; d++) h->tile[d] = t[d];
+  }
 
   // links (HexMesh::buildLinks)
   h->cells.assign(mesh->cell_nodes, mesh->cell_nodes + (size_t)nc * 8);
@@ -660,9 +744,9 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
       cz[c] = s[2] / 8;
     }
     double clo[3] = {lo[0] + 0.5 * hmin[0], lo[1] + 0.5 * hmin[1], lo[2] + 0.5 * hmin[2]};
-    h->cell_of = locality_order(nc, cx, cy, cz, hmin, clo);
+    h->cell_of = locality_order(nc, cx, cy, cz, hmin, clo, h->tile);
     std::vector<double> nx(X[0], X[0] + nn), ny(X[1], X[1] + nn), nz(X[2], X[2] + nn);
-    h->node_of = locality_order(nn, nx, ny, nz, hmin, lo);
+    h->node_of = locality_order(nn, nx, ny, nz, hmin, lo, h->tile);
   }
   h->cell_at.assign(nc, -1);
   h->node_at.assign(nn, -1);
@@ -670,6 +754,13 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
   for (int32_t i = 0; i < nn; i++) h->node_at[h->node_of[i]] = i;
 
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  size_t lds_limit = 0;
+  {
+    int dev = 0, bytes = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    lds_limit = (size_t)std::min(bytes, 64 * 1024);
+  }
 
   // node arrays
   {
@@ -697,6 +788,17 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
     void* p;
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->ncell = (int32_t*)p;
+    LdsPlan plan;
+    if (h->lds && build_lds_plan(conn, nn, h->block_v, plan) &&
+        (size_t)6 * plan.max_u * h->elt() <= lds_limit) {
+      if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
+      h->v_uoff = (int32_t*)p;
+      if ((rc = upload(h.get(), &p, plan.ulist))) return rc;
+      h->v_ulist = (int32_t*)p;
+      if ((rc = upload(h.get(), &p, plan.lconn))) return rc;
+      h->v_lconn = (uint16_t*)p;
+      h->v_maxu = plan.max_u;
+    }
     if ((rc = upload(h.get(), &p, w))) return rc;
     h->node_w = (double*)p;
   }
@@ -705,8 +807,9 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
     std::vector<double> dc[3], w(nc);
     for (int d = 0; d < 3; d++) dc[d].resize(nc);
     std::vector<int32_t> conn((size_t)nc * 8);
-    h->ldc = ((int64_t)nc + 63) / 64 * 64;
-    std::vector<double> C((size_t)21 * h->ldc, 0.0);
+    const int64_t ncp = ((int64_t)nc + 63) / 64 * 64;
+    h->ldc = h->c_blocked ? 0 : ncp;
+    std::vector<double> C((size_t)21 * ncp, 0.0);
     for (int32_t i = 0; i < nc; i++) {
       const int32_t c = h->cell_of[i];
       for (int d = 0; d < 3; d++)
@@ -715,7 +818,8 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
       for (int k = 0; k < 8; k++) conn[(size_t)i * 8 + k] = h->node_at[h->cells[(size_t)c * 8 + k]];
       for (int j = 0; j < 21; j++) {
         const double cij = field_at_cell(model->cij[j], h->cells.data(), c);
-        C[(size_t)j * h->ldc + i] = precision == RSG_FP64 ? cij : h->dt * cij;
+        const size_t idx = h->c_blocked ? ((size_t)(i >> 6) * 21 + j) * 64 + (i & 63) : (size_t)j * ncp + i;
+        C[idx] = precision == RSG_FP64 ? cij : h->dt * cij;
       }
     }
     h->dcx_h = dc[0];
@@ -725,6 +829,17 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
     void* p;
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->cnode = (int32_t*)p;
+    LdsPlan plan;
+    if (h->lds && build_lds_plan(conn, nc, h->block_s, plan) &&
+        (size_t)3 * plan.max_u * h->elt() <= lds_limit) {
+      if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
+      h->s_uoff = (int32_t*)p;
+      if ((rc = upload(h.get(), &p, plan.ulist))) return rc;
+      h->s_ulist = (int32_t*)p;
+      if ((rc = upload(h.get(), &p, plan.lconn))) return rc;
+      h->s_lconn = (uint16_t*)p;
+      h->s_maxu = plan.max_u;
+    }
     if ((rc = upload(h.get(), &p, w))) return rc;
     h->cell_w = (double*)p;
   }

@@ -32,6 +32,12 @@ struct VelocityArgs {
   const T* dnz;
   const int32_t* ncell;  // [n][8] connected cells by slot, -1 = none
   int32_t n;
+  int32_t xcd;  // 1: remap workgroups so each XCD owns a contiguous range
+  // LDS staging (lconn != nullptr): per-block unique cell lists and 16-bit local links
+  const int32_t* u_off;    // [nblocks+1]
+  const int32_t* ulist;    // cells staged by each block
+  const uint16_t* lconn;   // [n][8] block-local cell index by slot, 0xFFFF = none
+  int32_t max_u;           // largest unique set of a block (LDS row length)
   // special nodes (sources and/or sensors)
   const T* sp_idt;         // [nsp] real dt/rho_n
   const int32_t* sp_off;   // [nsp][4] term ranges for directions 0,1,2
@@ -53,12 +59,17 @@ struct StressArgs {
   const T* V1;
   const T* V2;
   const T* C;    // [21][ldc]: FP64 C_IJ; FP32 dt*C_IJ (FP64 product, rounded)
-  int64_t ldc;   // leading dimension of C (>= n)
+  int64_t ldc;   // leading dimension of C (>= n), or 0 for the 64-cell blocked layout
   const T* dcx;  // FP64: cell spacing dxc (NaN-boxed for special cells); FP32: 0.25/dxc
   const T* dcy;
   const T* dcz;
   const int32_t* cnode;  // [n][8] vertices
   int32_t n;
+  int32_t xcd;
+  const int32_t* u_off;    // LDS staging: per-block unique node lists
+  const int32_t* ulist;
+  const uint16_t* lconn;   // [n][8] block-local vertex index
+  int32_t max_u;
   T dt;  // used by the FP64 form only
   // special cells (stress sources)
   const T* sp_dcx;        // [nsp] real x-spacing value

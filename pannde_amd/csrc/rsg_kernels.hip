@@ -24,7 +24,37 @@
 // both computed in FP64 on the host and rounded once.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "rsg_internal.h"
+
+// Occupancy hints (minimum waves per SIMD, 0 = compiler default) and whether the stress
+// kernel requests its 21 C coefficients before the LDS staging (1) or after it (0).
+// Largest workgroup the gather kernels accept.  It bounds the registers the compiler may
+// give a thread (512 threads -> 2 waves per SIMD -> up to 256 VGPRs); 1024 would cap them
+// at 128, which makes the FP64 stress kernel spill to scratch.
+#ifndef RSG_MAX_BLOCK
+#define RSG_MAX_BLOCK 512
+#endif
+#ifndef RSG_STRESS_WPE
+#define RSG_STRESS_WPE 0
+#endif
+#ifndef RSG_VELOCITY_WPE
+#define RSG_VELOCITY_WPE 0
+#endif
+#ifndef RSG_STRESS_EARLY_C
+#define RSG_STRESS_EARLY_C 1
+#endif
+#if RSG_STRESS_WPE > 0
+#define RSG_STRESS_ATTR __attribute__((amdgpu_waves_per_eu(RSG_STRESS_WPE)))
+#else
+#define RSG_STRESS_ATTR
+#endif
+#if RSG_VELOCITY_WPE > 0
+#define RSG_VELOCITY_ATTR __attribute__((amdgpu_waves_per_eu(RSG_VELOCITY_WPE)))
+#else
+#define RSG_VELOCITY_ATTR
+#endif
 
 namespace rsg {
 namespace {
@@ -99,30 +129,108 @@ __device__ __forceinline__ void load_conn(const int32_t* __restrict__ conn, int 
   c[4] = hi.x; c[5] = hi.y; c[6] = hi.z; c[7] = hi.w;
 }
 
-template <typename T>
-__global__ __launch_bounds__(1024) void velocity_kernel(VelocityArgs<T> a) {
-  const int n = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (n >= a.n) return;
+// 8 x uint16 block-local indices of one entity as one 16-byte load (0xFFFF = none).
+__device__ __forceinline__ void load_lconn(const uint16_t* __restrict__ lconn, int i, int (&c)[8]) {
+  const v4i w = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(lconn) + (size_t)i);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t u = (uint32_t)w[k];
+    const int lo = (int)(u & 0xFFFFu), hi = (int)(u >> 16);
+    c[2 * k] = lo == 0xFFFF ? -1 : lo;
+    c[2 * k + 1] = hi == 0xFFFF ? -1 : hi;
+  }
+}
+
+// Stages NCOMP fields of the block's unique gather set (ulist[u_off[b]..u_off[b+1]))
+// into LDS as [NCOMP][max_u], each value read from HBM once per block.  With U > 0 every
+// thread owns at most U entries and all their loads are issued before any wait (two
+// dependent round trips in all: the list, then the values); U = 0 is the general loop.
+template <typename T, int NCOMP, int U>
+__device__ __forceinline__ void stage(T* lds, int max_u, const int32_t* __restrict__ u_off,
+                                      const int32_t* __restrict__ ulist, int b,
+                                      const T* const (&src)[NCOMP]) {
+  const int u0 = u_off[b], nu = u_off[b + 1] - u0;
+  if constexpr (U > 0) {
+    int g[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      g[u] = __builtin_nontemporal_load(ulist + u0 + (i < nu ? i : 0));
+    }
+    T v[U][NCOMP];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int k = 0; k < NCOMP; k++) v[u][k] = src[k][g[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = threadIdx.x + u * blockDim.x;
+      if (i < nu)
+#pragma unroll
+        for (int k = 0; k < NCOMP; k++) lds[k * max_u + i] = v[u][k];
+    }
+  } else {
+    for (int i = threadIdx.x; i < nu; i += blockDim.x) {
+      const int g = __builtin_nontemporal_load(ulist + u0 + i);
+#pragma unroll
+      for (int k = 0; k < NCOMP; k++) lds[k * max_u + i] = src[k][g];
+    }
+  }
+  __syncthreads();
+}
+
+// LDS: gathers come from the block's staged copy through 16-bit local indices;
+// otherwise directly from HBM through the int32 connectivity.
+template <typename T, bool LDS, int U>
+__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_kernel(VelocityArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int n0 = b * blockDim.x + threadIdx.x;
+  const bool live = n0 < a.n;
+  const int n = live ? n0 : a.n - 1;  // tail threads load a valid node and discard it
+  // streaming operands first (see stress_kernel)
   int c[8];
-  load_conn(a.ncell, n, c);
+  if constexpr (LDS)
+    load_lconn(a.lconn, n, c);
+  else
+    load_conn(a.ncell, n, c);
   T idt = a.idt[n];
   const T sx = a.dnx[n], sy = a.dny[n], sz = a.dnz[n];
+  const T vold0 = a.V0[n], vold1 = a.V1[n], vold2 = a.V2[n];
+  const T* S0 = a.S0;
+  const T* S1 = a.S1;
+  const T* S2 = a.S2;
+  const T* S3 = a.S3;
+  const T* S4 = a.S4;
+  const T* S5 = a.S5;
+  if constexpr (LDS) {
+    T* lds = reinterpret_cast<T*>(smem);
+    const T* const src[6] = {a.S0, a.S1, a.S2, a.S3, a.S4, a.S5};
+    stage<T, 6, U>(lds, a.max_u, a.u_off, a.ulist, b, src);
+    S0 = lds;
+    S1 = lds + a.max_u;
+    S2 = lds + 2 * a.max_u;
+    S3 = lds + 3 * a.max_u;
+    S4 = lds + 4 * a.max_u;
+    S5 = lds + 5 * a.max_u;
+  }
+  if (!live) return;
 
   T f[8];
-  load8_cells(a.S0, c, f);
+  load8_cells(S0, c, f);
   const T dx_sxx = dscale(diffsum<0>(f), sx);
-  load8_cells(a.S5, c, f);
+  load8_cells(S5, c, f);
   const T dx_sxy = dscale(diffsum<0>(f), sx);
   const T dy_sxy = dscale(diffsum<1>(f), sy);
-  load8_cells(a.S4, c, f);
+  load8_cells(S4, c, f);
   const T dx_sxz = dscale(diffsum<0>(f), sx);
   const T dz_sxz = dscale(diffsum<2>(f), sz);
-  load8_cells(a.S1, c, f);
+  load8_cells(S1, c, f);
   const T dy_syy = dscale(diffsum<1>(f), sy);
-  load8_cells(a.S3, c, f);
+  load8_cells(S3, c, f);
   const T dy_syz = dscale(diffsum<1>(f), sy);
   const T dz_syz = dscale(diffsum<2>(f), sz);
-  load8_cells(a.S2, c, f);
+  load8_cells(S2, c, f);
   const T dz_szz = dscale(diffsum<2>(f), sz);
 
   T r0 = (dx_sxx + dy_sxy) + dz_sxz;
@@ -145,9 +253,9 @@ __global__ __launch_bounds__(1024) void velocity_kernel(VelocityArgs<T> a) {
     r1 = r1 + (T)fsum[1];
     r2 = r2 + (T)fsum[2];
   }
-  const T v0 = a.V0[n] + idt * r0;
-  const T v1 = a.V1[n] + idt * r1;
-  const T v2 = a.V2[n] + idt * r2;
+  const T v0 = vold0 + idt * r0;
+  const T v1 = vold1 + idt * r1;
+  const T v2 = vold2 + idt * r2;
   a.V0[n] = v0;
   a.V1[n] = v1;
   a.V2[n] = v2;
@@ -163,7 +271,10 @@ __global__ __launch_bounds__(1024) void velocity_kernel(VelocityArgs<T> a) {
 
 template <typename T>
 __device__ __forceinline__ T cload(const T* __restrict__ C, int64_t ldc, int j, int i) {
-  return __builtin_nontemporal_load(C + (int64_t)j * ldc + i);
+  // ldc > 0: [21][ldc] arrays; ldc == 0: blocks of 64 cells, [n/64][21][64], so one
+  // wavefront's 21 coefficients are a single contiguous run
+  const int64_t idx = ldc ? (int64_t)j * ldc + i : ((int64_t)(i >> 6) * 21 + j) * 64 + (i & 63);
+  return __builtin_nontemporal_load(C + idx);
 }
 
 // Index of C_(I,J) (0-based Voigt indices) in the 21-entry upper triangle.
@@ -171,14 +282,45 @@ __host__ __device__ constexpr int tri(int i, int j) {
   return (i <= j) ? (i * 6 - i * (i - 1) / 2 + (j - i)) : (j * 6 - j * (j - 1) / 2 + (i - j));
 }
 
-template <typename T>
-__global__ __launch_bounds__(1024) void stress_kernel(StressArgs<T> a) {
-  const int e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (e >= a.n) return;
+template <typename T, bool LDS, int U>
+__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(StressArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int e0 = b * blockDim.x + threadIdx.x;
+  const bool live = e0 < a.n;
+  const int e = live ? e0 : a.n - 1;  // tail threads load a valid cell and discard it
+  // Every operand that does not depend on a gather is requested first, so the whole
+  // cell's streaming traffic (links, spacings, 21 C, 6 S) is in flight at once and its
+  // latency overlaps the LDS staging below.
   int c[8];
-  load_conn(a.cnode, e, c);
+  if constexpr (LDS)
+    load_lconn(a.lconn, e, c);
+  else
+    load_conn(a.cnode, e, c);
   T sx = a.dcx[e];
   const T sy = a.dcy[e], sz = a.dcz[e];
+  T cc[21];
+  if (RSG_STRESS_EARLY_C) {
+#pragma unroll
+    for (int j = 0; j < 21; j++) cc[j] = cload(a.C, a.ldc, j, e);
+  }
+  T s[6] = {a.S0[e], a.S1[e], a.S2[e], a.S3[e], a.S4[e], a.S5[e]};
+  const T* V0 = a.V0;
+  const T* V1 = a.V1;
+  const T* V2 = a.V2;
+  if constexpr (LDS) {
+    T* lds = reinterpret_cast<T*>(smem);
+    const T* const src[3] = {a.V0, a.V1, a.V2};
+    stage<T, 3, U>(lds, a.max_u, a.u_off, a.ulist, b, src);
+    V0 = lds;
+    V1 = lds + a.max_u;
+    V2 = lds + 2 * a.max_u;
+  }
+  if (!live) return;
+  if (!RSG_STRESS_EARLY_C) {
+#pragma unroll
+    for (int j = 0; j < 21; j++) cc[j] = cload(a.C, a.ldc, j, e);
+  }
   int sp = -1;
   if (is_boxed(sx)) {
     sp = unbox(sx);
@@ -186,15 +328,15 @@ __global__ __launch_bounds__(1024) void stress_kernel(StressArgs<T> a) {
   }
   // delV[i][k] = D_i V_k (i: derivative axis, k: velocity component)
   T f[8];
-  load8_nodes(a.V0, c, f);
+  load8_nodes(V0, c, f);
   const T d00 = dscale(diffsum<0>(f), sx);
   const T d10 = dscale(diffsum<1>(f), sy);
   const T d20 = dscale(diffsum<2>(f), sz);
-  load8_nodes(a.V1, c, f);
+  load8_nodes(V1, c, f);
   const T d01 = dscale(diffsum<0>(f), sx);
   const T d11 = dscale(diffsum<1>(f), sy);
   const T d21 = dscale(diffsum<2>(f), sz);
-  load8_nodes(a.V2, c, f);
+  load8_nodes(V2, c, f);
   const T d02 = dscale(diffsum<0>(f), sx);
   const T d12 = dscale(diffsum<1>(f), sy);
   const T d22 = dscale(diffsum<2>(f), sz);
@@ -211,26 +353,24 @@ __global__ __launch_bounds__(1024) void stress_kernel(StressArgs<T> a) {
     }
   }
 
-  T* Sout[6] = {a.S0, a.S1, a.S2, a.S3, a.S4, a.S5};
 #pragma unroll
   for (int ks = 0; ks < 6; ks++) {
-    const T c0 = cload(a.C, a.ldc, tri(ks, 0), e);
-    const T c1 = cload(a.C, a.ldc, tri(ks, 1), e);
-    const T c2 = cload(a.C, a.ldc, tri(ks, 2), e);
-    const T c3 = cload(a.C, a.ldc, tri(ks, 3), e);
-    const T c4 = cload(a.C, a.ldc, tri(ks, 4), e);
-    const T c5 = cload(a.C, a.ldc, tri(ks, 5), e);
-    T acc = c0 * d00 + c1 * d11 + c2 * d22 + c3 * e12 + c4 * e02 + c5 * e01;
-    T s = Sout[ks][e];
+    T acc = cc[tri(ks, 0)] * d00 + cc[tri(ks, 1)] * d11 + cc[tri(ks, 2)] * d22 +
+            cc[tri(ks, 3)] * e12 + cc[tri(ks, 4)] * e02 + cc[tri(ks, 5)] * e01;
     if constexpr (sizeof(T) == 8) {
       if (sp >= 0) acc = acc + (T)m[ks];
-      s = s + a.dt * acc;  // C is plain C_IJ: S += dt*(C:gradV + m)
+      s[ks] = s[ks] + a.dt * acc;  // C is plain C_IJ: S += dt*(C:gradV + m)
     } else {
       if (sp >= 0) acc = acc + (T)((double)a.dt * m[ks]);  // C holds dt*C_IJ
-      s = s + acc;
+      s[ks] = s[ks] + acc;
     }
-    Sout[ks][e] = s;
   }
+  a.S0[e] = s[0];
+  a.S1[e] = s[1];
+  a.S2[e] = s[2];
+  a.S3[e] = s[3];
+  a.S4[e] = s[4];
+  a.S5[e] = s[5];
 }
 
 // Energy: one FP64 partial per block for kinetic and stress-norm sums.
@@ -268,27 +408,66 @@ __global__ __launch_bounds__(256) void energy_kernel(EnergyArgs<T> a) {
 }
 
 template <typename Args, typename K>
-hipError_t launch1d(K kernel, const Args& a, int n, int block, hipStream_t s) {
+hipError_t launch1d(K kernel, const Args& a, int n, int block, size_t lds, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (block < 64 || block > 1024 || (block % 64) != 0) return hipErrorInvalidValue;
+  if (block < 64 || block > RSG_MAX_BLOCK || (block % 64) != 0) return hipErrorInvalidValue;
   const int grid = (n + block - 1) / block;
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s, a);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, s, a);
   return hipGetLastError();
+}
+
+// Picks the kernel instance: direct gathers, or LDS staging with U = ceil(max_u/block)
+// entries per thread (general loop beyond 4).
+template <typename T, typename Args, typename F>
+hipError_t launch_gather(const Args& a, int block, int ncomp, hipStream_t s, F kernel_for) {
+  if (!a.lconn) return launch1d(kernel_for(std::integral_constant<int, -1>()), a, a.n, block, 0, s);
+  const size_t lds = (size_t)ncomp * a.max_u * sizeof(T);
+  const int u = (a.max_u + block - 1) / block;
+  switch (u) {
+    case 0:
+    case 1: return launch1d(kernel_for(std::integral_constant<int, 1>()), a, a.n, block, lds, s);
+    case 2: return launch1d(kernel_for(std::integral_constant<int, 2>()), a, a.n, block, lds, s);
+    case 3: return launch1d(kernel_for(std::integral_constant<int, 3>()), a, a.n, block, lds, s);
+    case 4: return launch1d(kernel_for(std::integral_constant<int, 4>()), a, a.n, block, lds, s);
+    default: return launch1d(kernel_for(std::integral_constant<int, 0>()), a, a.n, block, lds, s);
+  }
+}
+
+template <typename T>
+hipError_t launch_velocity(const VelocityArgs<T>& a, int block, hipStream_t s) {
+  return launch_gather<T>(a, block, 6, s, [](auto u) {
+    constexpr int U = decltype(u)::value;
+    if constexpr (U < 0)
+      return velocity_kernel<T, false, 0>;
+    else
+      return velocity_kernel<T, true, U>;
+  });
+}
+
+template <typename T>
+hipError_t launch_stress(const StressArgs<T>& a, int block, hipStream_t s) {
+  return launch_gather<T>(a, block, 3, s, [](auto u) {
+    constexpr int U = decltype(u)::value;
+    if constexpr (U < 0)
+      return stress_kernel<T, false, 0>;
+    else
+      return stress_kernel<T, true, U>;
+  });
 }
 
 }  // namespace
 
 hipError_t launch_velocity_f64(const VelocityArgs<double>& a, int block, hipStream_t s) {
-  return launch1d(velocity_kernel<double>, a, a.n, block, s);
+  return launch_velocity<double>(a, block, s);
 }
 hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStream_t s) {
-  return launch1d(velocity_kernel<float>, a, a.n, block, s);
+  return launch_velocity<float>(a, block, s);
 }
 hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t s) {
-  return launch1d(stress_kernel<double>, a, a.n, block, s);
+  return launch_stress<double>(a, block, s);
 }
 hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t s) {
-  return launch1d(stress_kernel<float>, a, a.n, block, s);
+  return launch_stress<float>(a, block, s);
 }
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(energy_kernel<double>, dim3(nblocks), dim3(256), 0, s, a);

@@ -20,7 +20,7 @@ import numpy as np
 from . import cases as _cases
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpannde_rsg.so")
+LIB_PATH = os.environ.get("PANNDE_RSG_LIB") or os.path.join(HERE, "libpannde_rsg.so")
 
 RSG_OK = 0
 RSG_FP64, RSG_FP32 = 0, 1

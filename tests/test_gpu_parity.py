@@ -58,12 +58,19 @@ def test_device_matches_reference_fixture(name):
     assert_same(run_device(case), (ref["V"], ref["S"], ref["time"], ref["traces"]))
 
 
-@pytest.mark.parametrize("block", [64, 256, 1024])
+@pytest.mark.parametrize("block", [64, 128, 256, 512])
 def test_block_size_does_not_change_results(block, monkeypatch):
     monkeypatch.setenv("RSG_BLOCK_VELOCITY", str(block))
     monkeypatch.setenv("RSG_BLOCK_STRESS", str(block))
     case, ref = load_golden("cfrp_holes_rcm")
     assert_same(run_device(case), (ref["V"], ref["S"], ref["time"], ref["traces"]))
+
+
+def test_unsupported_block_size_is_rejected(monkeypatch):
+    monkeypatch.setenv("RSG_BLOCK_STRESS", "1024")
+    case, _ = load_golden("al_block_cellrho")
+    with pytest.raises(rsg.RSGError, match="workgroup size"):
+        rsg.DeviceSolver.from_case(case)
 
 
 @pytest.mark.parametrize("seed", [201, 202, 203, 204])
