@@ -1,0 +1,13 @@
+// PanNDE::Univariate: a function of time (reference modules/PanNDE/include/Univariate.hpp:69-81;
+// same signature, written for this build).
+#pragma once
+
+namespace PanNDE {
+
+class Univariate {
+ public:
+  virtual double evalAt(double t) = 0;
+  virtual ~Univariate() {}
+};
+
+}  // namespace PanNDE

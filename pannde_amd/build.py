@@ -44,6 +44,11 @@ def build(verbose=False):
     # the HBM copy probe (bandwidth ceiling and counter calibration), a separate program
     _run([HIPCC] + DEVICE_FLAGS[:5] + DEVICE_FLAGS[5:] + [os.path.join(CSRC, "hbm_copy_probe.hip"), "-o",
                                                           os.path.join(HERE, "hbm_copy_probe")], verbose)
+    # the C++ drop-in driver (PanNDE::Model over the C ABI), linked against the library
+    inc = os.path.join(ROOT, "include")
+    _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + inc, "-I" + os.path.join(inc, "PanNDE"),
+          os.path.join(CSRC, "rsg_driver.cpp"), "-o", os.path.join(HERE, "rsg_driver"), "-L" + HERE,
+          "-lpannde_rsg", "-Wl,-rpath,$ORIGIN"], verbose)
     return OUT
 
 
