@@ -113,6 +113,26 @@ typedef struct {
 int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t precision,
                rsg_handle* out);
 
+/* Kernel and layout choices of a handle (they never change results: every choice is
+ * bit-identical).  A zero field takes the default shown. */
+typedef struct {
+  int32_t block_velocity;      /* workgroup size of the velocity kernel, 64..512 step 64 (256) */
+  int32_t block_stress;        /* workgroup size of the stress kernel (256) */
+  int32_t per_thread_velocity; /* nodes per thread: 1, 2 or 4 (1) */
+  int32_t per_thread_stress;   /* cells per thread: 1, 2 or 4 (1) */
+  int32_t lds;                 /* 1: stage each workgroup's gathers in LDS, -1: gather from HBM (1) */
+  int32_t tile[3];             /* locality brick in cells along x, y, z (8, 8, 4) */
+  int32_t c_layout;            /* 1: C in 64-cell blocks [n/64][21][64], -1: [21][n] (1) */
+  int32_t xcd_remap;           /* 1: XCD-contiguous workgroup order, -1: hardware order (1) */
+} rsg_tuning;
+
+/* rsg_create with explicit kernel and layout choices (NULL tuning = defaults). */
+int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t precision,
+                  const rsg_tuning* tuning, rsg_handle* out);
+
+/* The choices a handle was created with, defaults filled in. */
+int rsg_get_tuning(rsg_handle h, rsg_tuning* out);
+
 /* Replaces the transducer list of HostRSGTDUT (setupTransducers, HostRSGTDUT.hpp:213-219)
  * and HostUTModel::applyTransducer / applyMomentExcitation (HostUTModel.hpp:138-170).
  * Replaces any previous channel set. */
@@ -170,6 +190,41 @@ int rsg_destroy(rsg_handle h);
 /* Copies the calling thread's last error message into buf (NUL-terminated); returns its
  * length. */
 int rsg_last_error(char* buf, size_t len);
+
+/* ---------------------------------------------------------------- batches
+ * Many independent instances advanced together: one persistent launch per kernel per
+ * step covers every instance of the batch (a device work table of (instance, block)
+ * items and a device table of per-instance arguments), and K consecutive steps are
+ * captured once in a HIP graph.  Instances join and leave between K-step blocks; each
+ * takes exactly its own number of steps.  No counterpart in the reference, which runs
+ * one model per process; each instance's arithmetic is that of rsg_step, so its results
+ * are bit-identical alone and in any mix.  Instances are ordinary handles (same
+ * precision, same workgroup sizes and per-thread counts), their channels must carry
+ * signal tables, and while in a batch they are stepped only by the batch; read their
+ * state and traces with rsg_get_state / rsg_get_traces after rsg_batch_synchronize. */
+typedef struct rsg_batch_s* rsg_batch;
+
+/* steps_per_block: K (0: 50).  grid: workgroups of the persistent launches (0: 8 per CU).
+ * use_graph: 0 launches every step directly instead of replaying a captured graph. */
+int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, int32_t use_graph,
+                     rsg_batch* out);
+
+/* Adds a handle that will take nsteps more steps; *slot receives its slot id. */
+int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot);
+
+/* Detaches a slot's handle (finished or not); the handle can be stepped alone again. */
+int rsg_batch_remove(rsg_batch b, int32_t slot);
+
+/* Runs up to nblocks K-step blocks (enqueued; see rsg_batch_synchronize).  *unfinished
+ * receives the number of instances with steps left. */
+int rsg_batch_run(rsg_batch b, int32_t nblocks, int32_t* unfinished);
+
+int rsg_batch_synchronize(rsg_batch b);
+
+/* Steps taken and requested of a slot's instance. */
+int rsg_batch_slot_status(rsg_batch b, int32_t slot, int64_t* done, int64_t* total);
+
+int rsg_batch_destroy(rsg_batch b);
 
 /* HostNormalXDZ::checkAt (modules/Stubs/include/HostNormalXDZ.hpp:140-145) over a mesh:
  * at = 0 tests every cell centre (HostRSGTDUT::cellcenter), at = 1 every node.  Writes

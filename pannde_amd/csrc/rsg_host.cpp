@@ -127,6 +127,9 @@ struct rsg_instance {
   double dt = 0, time = 0;
   int64_t steps = 0;
   int block_v = 256, block_s = 256;
+  int cpt_v = 1, cpt_s = 1;       // entities per thread
+  bool in_batch = false;          // stepped by a batch, not by rsg_step
+  hipStream_t batch_stream = nullptr;  // the batch's stream while in_batch
   int xcd = 1;                    // XCD-contiguous workgroup remap
   int lds = 1;                    // stage gathers through LDS when the block sets fit
   int c_blocked = 1;              // C as [n/64][21][64] instead of [21][n]
@@ -504,6 +507,7 @@ int flush_traces(rsg_instance* h) {
   const size_t old = h->trace_host.size();
   h->trace_host.resize(old + (size_t)h->trace_dev_steps * row);
   HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->batch_stream) HIP_TRY(hipStreamSynchronize(h->batch_stream));
   HIP_TRY(hipMemcpy(h->trace_host.data() + old, h->trace.p,
                     (size_t)h->trace_dev_steps * row * sizeof(double), hipMemcpyDeviceToHost));
   h->trace_dev_steps = 0;
@@ -528,6 +532,7 @@ void fill_args(rsg_instance* h, rsg::VelocityArgs<T>& va, rsg::StressArgs<T>& sa
   va.ncell = h->ncell;
   va.n = h->nn;
   va.xcd = h->xcd;
+  va.cpt = h->cpt_v;
   va.u_off = h->v_uoff;
   va.ulist = h->v_ulist;
   va.lconn = h->v_lconn;
@@ -553,6 +558,7 @@ void fill_args(rsg_instance* h, rsg::VelocityArgs<T>& va, rsg::StressArgs<T>& sa
   sa.cnode = h->cnode;
   sa.n = h->nc;
   sa.xcd = h->xcd;
+  sa.cpt = h->cpt_s;
   sa.u_off = h->s_uoff;
   sa.ulist = h->s_ulist;
   sa.lconn = h->s_lconn;
@@ -645,6 +651,24 @@ int rsg_last_error(char* buf, size_t len) {
 
 int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t precision,
                rsg_handle* out) {
+  return rsg_create_ex(mesh, model, precision, nullptr, out);
+}
+
+int rsg_get_tuning(rsg_handle h, rsg_tuning* t) {
+  if (!h || !t) return fail(RSG_ERR_ARG, "null argument");
+  t->block_velocity = h->block_v;
+  t->block_stress = h->block_s;
+  t->per_thread_velocity = h->cpt_v;
+  t->per_thread_stress = h->cpt_s;
+  t->lds = h->lds ? 1 : -1;
+  for (int d = 0; d < 3; d++) t->tile[d] = h->tile[d];
+  t->c_layout = h->c_blocked ? 1 : -1;
+  t->xcd_remap = h->xcd ? 1 : -1;
+  return RSG_OK;
+}
+
+int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t precision,
+                  const rsg_tuning* tuning, rsg_handle* out) {
   if (!out) return fail(RSG_ERR_ARG, "out is null");
   *out = nullptr;
   int rc = check_mesh(mesh);
@@ -665,19 +689,25 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
   h->nc = nc;
   h->dt = model->dt;
   h->moment_form = model->moment_form ? 1 : 0;
-  if (const char* e = getenv("RSG_BLOCK_VELOCITY")) h->block_v = atoi(e);
-  if (const char* e = getenv("RSG_BLOCK_STRESS")) h->block_s = atoi(e);
+  if (tuning) {
+    const rsg_tuning& t = *tuning;
+    if (t.block_velocity) h->block_v = t.block_velocity;
+    if (t.block_stress) h->block_s = t.block_stress;
+    if (t.per_thread_velocity) h->cpt_v = t.per_thread_velocity;
+    if (t.per_thread_stress) h->cpt_s = t.per_thread_stress;
+    if (t.lds) h->lds = t.lds > 0;
+    if (t.c_layout) h->c_blocked = t.c_layout > 0;
+    if (t.xcd_remap) h->xcd = t.xcd_remap > 0;
+    for (int d = 0; d < 3; d++) {
+      if (t.tile[d] < 0) return fail(RSG_ERR_ARG, "tile[%d] = %d is negative", d, t.tile[d]);
+      if (t.tile[d]) h->tile[d] = t.tile[d];
+    }
+  }
   for (int blk : {h->block_v, h->block_s})
     if (blk < 64 || blk > 512 || blk % 64 != 0)
       return fail(RSG_ERR_ARG, "workgroup size %d: must be a multiple of 64 in [64, 512]", blk);
-  if (const char* e = getenv("RSG_XCD_REMAP")) h->xcd = atoi(e) != 0;
-  if (const char* e = getenv("RSG_LDS")) h->lds = atoi(e) != 0;
-  if (const char* e = getenv("RSG_C_BLOCKED")) h->c_blocked = atoi(e) != 0;
-  if (const char* e = getenv("RSG_TILE")) {
-    int t[3];
-    if (sscanf(e, "%d,%d,%d", &t[0], &t[1], &t[2]) == 3 && t[0] > 0 && t[1] > 0 && t[2] > 0)
-      for (int d = 0; d < 3; d++) h->tile[d] = t[d];
-  }
+  for (int c : {h->cpt_v, h->cpt_s})
+    if (c != 1 && c != 2 && c != 4) return fail(RSG_ERR_ARG, "entities per thread %d: must be 1, 2 or 4", c);
 
   // links (HexMesh::buildLinks)
   h->cells.assign(mesh->cell_nodes, mesh->cell_nodes + (size_t)nc * 8);
@@ -789,7 +819,7 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->ncell = (int32_t*)p;
     LdsPlan plan;
-    if (h->lds && build_lds_plan(conn, nn, h->block_v, plan) &&
+    if (h->lds && build_lds_plan(conn, nn, h->block_v * h->cpt_v, plan) &&
         (size_t)6 * plan.max_u * h->elt() <= lds_limit) {
       if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
       h->v_uoff = (int32_t*)p;
@@ -830,7 +860,7 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->cnode = (int32_t*)p;
     LdsPlan plan;
-    if (h->lds && build_lds_plan(conn, nc, h->block_s, plan) &&
+    if (h->lds && build_lds_plan(conn, nc, h->block_s * h->cpt_s, plan) &&
         (size_t)3 * plan.max_u * h->elt() <= lds_limit) {
       if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
       h->s_uoff = (int32_t*)p;
@@ -860,6 +890,7 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
 
 int rsg_set_channels(rsg_handle h, int32_t n, const rsg_channel_desc* ch) {
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (h->in_batch) return fail(RSG_ERR_STATE, "the handle is in a batch; remove it first");
   if (n < 0 || (n > 0 && !ch)) return fail(RSG_ERR_ARG, "bad channel list");
   std::vector<Channel> list((size_t)n);
   for (int32_t k = 0; k < n; k++) {
@@ -897,6 +928,7 @@ int rsg_set_channels(rsg_handle h, int32_t n, const rsg_channel_desc* ch) {
 
 int rsg_set_sensors(rsg_handle h, int32_t n, const int32_t* nodes) {
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (h->in_batch) return fail(RSG_ERR_STATE, "the handle is in a batch; remove it first");
   if (n < 0 || (n > 0 && !nodes)) return fail(RSG_ERR_ARG, "bad sensor list");
   for (int32_t s = 0; s < n; s++)
     if (nodes[s] < 0 || nodes[s] >= h->nn) return fail(RSG_ERR_ARG, "sensor node %d out of range", nodes[s]);
@@ -917,6 +949,7 @@ int rsg_set_sensors(rsg_handle h, int32_t n, const int32_t* nodes) {
 
 int rsg_step(rsg_handle h, int32_t nsteps, const double* amplitudes, double* time_out) {
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (h->in_batch) return fail(RSG_ERR_STATE, "the handle is stepped by a batch; remove it first");
   if (nsteps < 0) return fail(RSG_ERR_ARG, "nsteps < 0");
   int rc;
   if (h->tables_dirty && (rc = build_tables(h))) return rc;
@@ -969,6 +1002,7 @@ int rsg_step(rsg_handle h, int32_t nsteps, const double* amplitudes, double* tim
 int rsg_get_state(rsg_handle h, double* V, double* S) {
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
   HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->batch_stream) HIP_TRY(hipStreamSynchronize(h->batch_stream));
   const size_t e = h->elt();
   std::vector<char> buf((size_t)std::max(h->nn, h->nc) * e);
   auto fetch = [&](void* src, int32_t n, const std::vector<int32_t>& of, double* dst) -> int {
@@ -994,6 +1028,7 @@ int rsg_get_state(rsg_handle h, double* V, double* S) {
 
 int rsg_set_state(rsg_handle h, const double* V, const double* S) {
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (h->in_batch) return fail(RSG_ERR_STATE, "the handle is in a batch; remove it first");
   HIP_TRY(hipStreamSynchronize(h->stream));
   const size_t e = h->elt();
   std::vector<char> buf((size_t)std::max(h->nn, h->nc) * e);
@@ -1035,6 +1070,7 @@ int rsg_get_traces(rsg_handle h, int64_t first, int64_t n, double* out) {
 int rsg_synchronize(rsg_handle h) {
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
   HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->batch_stream) HIP_TRY(hipStreamSynchronize(h->batch_stream));
   return RSG_OK;
 }
 
@@ -1047,6 +1083,7 @@ int rsg_get_time(rsg_handle h, double* time, int64_t* steps) {
 
 int rsg_energy(rsg_handle h, double* kinetic, double* stress_norm) {
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (h->batch_stream) HIP_TRY(hipStreamSynchronize(h->batch_stream));
   const int nb = 1024;
   hipError_t e;
   if (h->precision == RSG_FP64) {
@@ -1115,6 +1152,7 @@ int rsg_device_bytes(rsg_handle h, int64_t* bytes) {
 }
 
 int rsg_destroy(rsg_handle h) {
+  if (h && h->in_batch) return fail(RSG_ERR_STATE, "the handle is in a batch; remove it first");
   delete h;
   return RSG_OK;
 }
@@ -1162,6 +1200,378 @@ int rsg_signal_eval(int32_t n, const double* t, const double* v, int32_t n_eval,
   s.build(n, t, v);
   for (int32_t i = 0; i < n_eval; i++)
     if (!s.eval(at[i], &out[i])) return fail(RSG_ERR_ARG, "time %g precedes the first key", at[i]);
+  return RSG_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+
+struct rsg_batch_s {
+  int32_t precision = RSG_FP64;
+  int K = 50;
+  int grid = 0;
+  bool use_graph = true;
+  hipStream_t stream = nullptr;
+  struct Slot {
+    rsg_instance* h = nullptr;
+    int64_t total = 0, done = 0;
+    bool attached = false;
+  };
+  std::vector<Slot> slots;
+  // kernel configuration shared by every instance
+  bool configured = false;
+  int block_v = 256, block_s = 256, cpt_v = 1, cpt_s = 1;
+  bool lds_v = true, lds_s = true;
+  int32_t max_u_v = 0, max_u_s = 0;
+  // control block on the device: descriptors | velocity work | stress work | amplitudes
+  size_t cap_inst = 0, cap_wv = 0, cap_ws = 0, cap_amp = 0;
+  size_t off_wv = 0, off_ws = 0, off_amp = 0, ctl_bytes = 0;
+  char* ctl_dev = nullptr;
+  char* ctl_host[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool busy[2] = {false, false};
+  int cur = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_dirty = true;
+
+  size_t desc_size() const {
+    return precision == RSG_FP64 ? sizeof(rsg::BatchDesc<double>) : sizeof(rsg::BatchDesc<float>);
+  }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_dirty = true;
+  }
+  ~rsg_batch_s() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& sl : slots)
+      if (sl.attached && sl.h) {
+        sl.h->in_batch = false;
+        sl.h->batch_stream = nullptr;
+      }
+    drop_graph();
+    if (ctl_dev) (void)hipFree(ctl_dev);
+    for (int i = 0; i < 2; i++) {
+      if (ctl_host[i]) (void)hipHostFree(ctl_host[i]);
+      if (ev[i]) (void)hipEventDestroy(ev[i]);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Grows the control block to hold the current instances (amplitude area for K steps).
+int batch_reserve(rsg_batch_s* b) {
+  size_t ninst = b->slots.size(), nwv = 0, nws = 0, namp = 0;
+  for (auto& sl : b->slots) {
+    if (!sl.attached) continue;
+    rsg_instance* h = sl.h;
+    nwv += (size_t)(h->nn + b->block_v * b->cpt_v - 1) / (b->block_v * b->cpt_v);
+    nws += (size_t)(h->nc + b->block_s * b->cpt_s - 1) / (b->block_s * b->cpt_s);
+    namp += std::max<size_t>(1, h->channels.size()) * (size_t)b->K;
+  }
+  if (ninst <= b->cap_inst && nwv <= b->cap_wv && nws <= b->cap_ws && namp <= b->cap_amp && b->ctl_dev)
+    return RSG_OK;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->cap_inst = std::max(b->cap_inst, ninst + ninst / 2 + 4);
+  b->cap_wv = std::max(b->cap_wv, nwv + nwv / 2 + 64);
+  b->cap_ws = std::max(b->cap_ws, nws + nws / 2 + 64);
+  b->cap_amp = std::max(b->cap_amp, namp + namp / 2 + 64);
+  b->off_wv = align256(b->cap_inst * b->desc_size());
+  b->off_ws = b->off_wv + align256((2 + 2 * b->cap_wv) * sizeof(int32_t));
+  b->off_amp = b->off_ws + align256((2 + 2 * b->cap_ws) * sizeof(int32_t));
+  b->ctl_bytes = b->off_amp + align256(b->cap_amp * sizeof(double));
+  if (b->ctl_dev) (void)hipFree(b->ctl_dev);
+  b->ctl_dev = nullptr;
+  HIP_TRY(hipMalloc((void**)&b->ctl_dev, b->ctl_bytes));
+  for (int i = 0; i < 2; i++) {
+    if (b->ctl_host[i]) (void)hipHostFree(b->ctl_host[i]);
+    b->ctl_host[i] = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&b->ctl_host[i], b->ctl_bytes, hipHostMallocDefault));
+    b->busy[i] = false;
+  }
+  b->drop_graph();  // the graph's kernels point into the control block
+  return RSG_OK;
+}
+
+template <typename T>
+int batch_launch_block(rsg_batch_s* b) {
+  const auto* desc = (const rsg::BatchDesc<T>*)b->ctl_dev;
+  const int32_t* wv = (const int32_t*)(b->ctl_dev + b->off_wv);
+  const int32_t* ws = (const int32_t*)(b->ctl_dev + b->off_ws);
+  auto enqueue = [&]() -> hipError_t {
+    for (int k = 0; k < b->K; k++) {
+      hipError_t e;
+      if (sizeof(T) == 8) {
+        e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv, k, b->grid, b->block_v, b->lds_v,
+                                  b->max_u_v, b->cpt_v, true, b->stream);
+        if (e != hipSuccess) return e;
+        e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws, k, b->grid, b->block_s, b->lds_s,
+                                  b->max_u_s, b->cpt_s, false, b->stream);
+      } else {
+        e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv, k, b->grid, b->block_v, b->lds_v,
+                                  b->max_u_v, b->cpt_v, true, b->stream);
+        if (e != hipSuccess) return e;
+        e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws, k, b->grid, b->block_s, b->lds_s,
+                                  b->max_u_s, b->cpt_s, false, b->stream);
+      }
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  };
+  if (!b->use_graph) {
+    hipError_t e = enqueue();
+    if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "batch launch: %s", hipGetErrorString(e));
+    return RSG_OK;
+  }
+  if (b->graph_dirty || !b->exec) {
+    b->drop_graph();
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue();
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(RSG_ERR_DEVICE, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_dirty = false;
+  }
+  HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  return RSG_OK;
+}
+
+// Fills one K-step block's control data (descriptors, work tables, amplitudes) for the
+// instances with steps left, uploads it and launches the block.
+template <typename T>
+int batch_block(rsg_batch_s* b, int32_t* unfinished) {
+  int rc;
+  if ((rc = batch_reserve(b))) return rc;
+  const int j = b->cur;
+  if (b->busy[j]) HIP_TRY(hipEventSynchronize(b->ev[j]));
+  char* host = b->ctl_host[j];
+  auto* desc = (rsg::BatchDesc<T>*)host;
+  int32_t* wv = (int32_t*)(host + b->off_wv);
+  int32_t* ws = (int32_t*)(host + b->off_ws);
+  double* amp = (double*)(host + b->off_amp);
+  int32_t nwv = 0, nws = 0;
+  size_t amp_off = 0;
+  // traces that would overflow their device buffer are flushed first (needs the stream idle)
+  bool flush = false;
+  for (auto& sl : b->slots)
+    if (sl.attached && sl.done < sl.total && !sl.h->sensors.empty() &&
+        sl.h->trace_dev_steps + b->K > kTraceCapSteps)
+      flush = true;
+  if (flush) {
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (auto& sl : b->slots)
+      if (sl.attached && (rc = flush_traces(sl.h))) return rc;
+  }
+  std::vector<int> kmax(b->slots.size(), 0);
+  for (size_t i = 0; i < b->slots.size(); i++) {
+    auto& sl = b->slots[i];
+    rsg::BatchDesc<T>& d = desc[i];
+    memset(&d, 0, sizeof(d));
+    if (!sl.attached || sl.done >= sl.total) continue;
+    rsg_instance* h = sl.h;
+    if (h->tables_dirty && (rc = build_tables(h))) return rc;
+    const int km = (int)std::min<int64_t>(b->K, sl.total - sl.done);
+    kmax[i] = km;
+    rsg::VelocityArgs<T> va;
+    rsg::StressArgs<T> sa;
+    fill_args<T>(h, va, sa);
+    d.v = va;
+    d.s = sa;
+    const size_t nch = std::max<size_t>(1, h->channels.size());
+    d.nch = (int32_t)nch;
+    d.amp_base = (const double*)(b->ctl_dev + b->off_amp) + amp_off;
+    d.row = (int32_t)(h->sensors.size() * 3);
+    d.trace_base = d.row ? h->trace.p : nullptr;
+    d.trace_row0 = h->trace_dev_steps;
+    d.kmax = km;
+    double t = h->time;
+    for (int k = 0; k < km; k++) {
+      for (size_t c = 0; c < h->channels.size(); c++) {
+        double sv;
+        if (!h->channels[c].sig.eval(t, &sv))
+          return fail(RSG_ERR_ARG, "channel %zu: time %g precedes its first signal time", c, t);
+        amp[amp_off + (size_t)k * nch + c] = 0.125 * (sv * 1.0);
+      }
+      t += h->dt;
+    }
+    amp_off += nch * (size_t)b->K;
+    const int32_t cv = b->block_v * b->cpt_v, cs = b->block_s * b->cpt_s;
+    const int32_t nbv = (h->nn + cv - 1) / cv, nbs = (h->nc + cs - 1) / cs;
+    for (int32_t q = 0; q < nbv; q++) {
+      wv[2 + 2 * nwv] = (int32_t)i;
+      wv[3 + 2 * nwv] = q;
+      nwv++;
+    }
+    for (int32_t q = 0; q < nbs; q++) {
+      ws[2 + 2 * nws] = (int32_t)i;
+      ws[3 + 2 * nws] = q;
+      nws++;
+    }
+  }
+  wv[0] = nwv;
+  wv[1] = 0;
+  ws[0] = nws;
+  ws[1] = 0;
+  if (nwv == 0 && nws == 0) {
+    *unfinished = 0;
+    return RSG_OK;
+  }
+  HIP_TRY(hipMemcpyAsync(b->ctl_dev, host, b->ctl_bytes, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipEventRecord(b->ev[j], b->stream));
+  b->busy[j] = true;
+  b->cur ^= 1;
+  if ((rc = batch_launch_block<T>(b))) return rc;
+  int32_t left = 0;
+  for (size_t i = 0; i < b->slots.size(); i++) {
+    auto& sl = b->slots[i];
+    if (!sl.attached || kmax[i] == 0) continue;
+    rsg_instance* h = sl.h;
+    for (int k = 0; k < kmax[i]; k++) h->time += h->dt;
+    h->steps += kmax[i];
+    if (!h->sensors.empty()) h->trace_dev_steps += kmax[i];
+    sl.done += kmax[i];
+    if (sl.done < sl.total) left++;
+  }
+  *unfinished = left;
+  return RSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, int32_t use_graph,
+                     rsg_batch* out) {
+  if (!out) return fail(RSG_ERR_ARG, "out is null");
+  *out = nullptr;
+  if (precision != RSG_FP64 && precision != RSG_FP32) return fail(RSG_ERR_ARG, "bad precision %d", precision);
+  if (steps_per_block < 0 || steps_per_block > 4096) return fail(RSG_ERR_ARG, "steps_per_block %d", steps_per_block);
+  if (grid < 0) return fail(RSG_ERR_ARG, "grid %d", grid);
+  std::unique_ptr<rsg_batch_s> b(new (std::nothrow) rsg_batch_s());
+  if (!b) return fail(RSG_ERR_NOMEM, "host allocation failed");
+  b->precision = precision;
+  b->K = steps_per_block ? steps_per_block : 50;
+  b->use_graph = use_graph != 0;
+  if (grid) {
+    b->grid = grid;
+  } else {
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    b->grid = std::max(1, cus) * 8;
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  for (int i = 0; i < 2; i++) HIP_TRY(hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming));
+  *out = b.release();
+  return RSG_OK;
+}
+
+int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot) {
+  if (!b || !h || !slot) return fail(RSG_ERR_ARG, "null argument");
+  if (nsteps < 0) return fail(RSG_ERR_ARG, "nsteps < 0");
+  if (h->in_batch) return fail(RSG_ERR_STATE, "the handle is already in a batch");
+  if (h->precision != b->precision) return fail(RSG_ERR_ARG, "precision differs from the batch's");
+  for (auto& c : h->channels)
+    if (!c.has_signal) return fail(RSG_ERR_ARG, "batched instances need signal tables on every channel");
+  if (b->configured) {
+    if (h->block_v != b->block_v || h->block_s != b->block_s || h->cpt_v != b->cpt_v || h->cpt_s != b->cpt_s)
+      return fail(RSG_ERR_ARG, "workgroup sizes / per-thread counts differ from the batch's");
+  } else {
+    b->block_v = h->block_v;
+    b->block_s = h->block_s;
+    b->cpt_v = h->cpt_v;
+    b->cpt_s = h->cpt_s;
+    b->lds_v = h->v_lconn != nullptr;
+    b->lds_s = h->s_lconn != nullptr;
+    b->configured = true;
+  }
+  int rc;
+  if (h->tables_dirty && (rc = build_tables(h))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  // one kernel variant serves the batch: LDS staging only if every instance has a plan,
+  // sized for the largest staged set
+  const bool lv = b->lds_v && h->v_lconn, ls = b->lds_s && h->s_lconn;
+  const int32_t mv = std::max(b->max_u_v, h->v_maxu), ms = std::max(b->max_u_s, h->s_maxu);
+  if (lv != b->lds_v || ls != b->lds_s || mv != b->max_u_v || ms != b->max_u_s) b->graph_dirty = true;
+  b->lds_v = lv;
+  b->lds_s = ls;
+  b->max_u_v = mv;
+  b->max_u_s = ms;
+  int32_t id = -1;
+  for (size_t i = 0; i < b->slots.size(); i++)
+    if (!b->slots[i].attached) {
+      id = (int32_t)i;
+      break;
+    }
+  if (id < 0) {
+    id = (int32_t)b->slots.size();
+    b->slots.emplace_back();
+  }
+  auto& sl = b->slots[id];
+  sl.h = h;
+  sl.total = nsteps;
+  sl.done = 0;
+  sl.attached = true;
+  h->in_batch = true;
+  h->batch_stream = b->stream;
+  *slot = id;
+  return RSG_OK;
+}
+
+int rsg_batch_remove(rsg_batch b, int32_t slot) {
+  if (!b || slot < 0 || slot >= (int32_t)b->slots.size() || !b->slots[slot].attached)
+    return fail(RSG_ERR_ARG, "no such slot %d", slot);
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  auto& sl = b->slots[slot];
+  sl.h->in_batch = false;
+  sl.h->batch_stream = nullptr;
+  sl = rsg_batch_s::Slot();
+  return RSG_OK;
+}
+
+int rsg_batch_run(rsg_batch b, int32_t nblocks, int32_t* unfinished) {
+  if (!b) return fail(RSG_ERR_ARG, "batch is null");
+  int32_t left = 0;
+  for (auto& sl : b->slots)
+    if (sl.attached && sl.done < sl.total) left++;
+  for (int32_t q = 0; q < nblocks && left > 0; q++) {
+    const int rc = b->precision == RSG_FP64 ? batch_block<double>(b, &left) : batch_block<float>(b, &left);
+    if (rc) return rc;
+  }
+  if (unfinished) *unfinished = left;
+  return RSG_OK;
+}
+
+int rsg_batch_synchronize(rsg_batch b) {
+  if (!b) return fail(RSG_ERR_ARG, "batch is null");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RSG_OK;
+}
+
+int rsg_batch_slot_status(rsg_batch b, int32_t slot, int64_t* done, int64_t* total) {
+  if (!b || slot < 0 || slot >= (int32_t)b->slots.size() || !b->slots[slot].attached)
+    return fail(RSG_ERR_ARG, "no such slot %d", slot);
+  if (done) *done = b->slots[slot].done;
+  if (total) *total = b->slots[slot].total;
+  return RSG_OK;
+}
+
+int rsg_batch_destroy(rsg_batch b) {
+  delete b;
   return RSG_OK;
 }
 

@@ -33,6 +33,7 @@ struct VelocityArgs {
   const int32_t* ncell;  // [n][8] connected cells by slot, -1 = none
   int32_t n;
   int32_t xcd;  // 1: remap workgroups so each XCD owns a contiguous range
+  int32_t cpt;  // entities per thread: a workgroup covers cpt * blockDim.x nodes
   // LDS staging (lconn != nullptr): per-block unique cell lists and 16-bit local links
   const int32_t* u_off;    // [nblocks+1]
   const int32_t* ulist;    // cells staged by each block
@@ -66,6 +67,7 @@ struct StressArgs {
   const int32_t* cnode;  // [n][8] vertices
   int32_t n;
   int32_t xcd;
+  int32_t cpt;             // cells per thread
   const int32_t* u_off;    // LDS staging: per-block unique node lists
   const int32_t* ulist;
   const uint16_t* lconn;   // [n][8] block-local vertex index
@@ -92,11 +94,33 @@ struct EnergyArgs {
   double* partial;  // [2][nblocks]
 };
 
+// One instance of a batch: its argument blocks plus where this K-step block's
+// amplitudes and traces go.
+template <typename T>
+struct BatchDesc {
+  VelocityArgs<T> v;
+  StressArgs<T> s;
+  const double* amp_base;  // [K][nch] amplitudes of the current K-step block
+  double* trace_base;      // the instance's trace buffer, [rows][row] (nullptr: no sensors)
+  int32_t nch;
+  int32_t row;             // 3 * sensors
+  int64_t trace_row0;      // first trace row of this K-step block
+  int32_t kmax;            // steps of this K-step block the instance takes
+  int32_t pad;
+};
+
 // Launchers (rsg_kernels.hip).  block = threads per block (64..1024, multiple of 64).
 hipError_t launch_velocity_f64(const VelocityArgs<double>& a, int block, hipStream_t s);
 hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStream_t s);
 hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t s);
 hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t s);
+// Batched persistent launch: grid workgroups walk work = {n, 0, (instance, block)...};
+// lds selects LDS staging (max_u: the largest staged set of any instance), cpt the
+// entities per thread (1, 2 or 4).
+hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* work, int k, int grid, int block,
+                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s);
+hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* work, int k, int grid, int block,
+                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s);
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s);
 hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nblocks, hipStream_t s);
 

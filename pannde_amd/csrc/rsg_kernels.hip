@@ -28,8 +28,7 @@
 
 #include "rsg_internal.h"
 
-// Occupancy hints (minimum waves per SIMD, 0 = compiler default) and whether the stress
-// kernel requests its 21 C coefficients before the LDS staging (1) or after it (0).
+// Occupancy hints (minimum waves per SIMD, 0 = compiler default).
 // Largest workgroup the gather kernels accept.  It bounds the registers the compiler may
 // give a thread (512 threads -> 2 waves per SIMD -> up to 256 VGPRs); 1024 would cap them
 // at 128, which makes the FP64 stress kernel spill to scratch.
@@ -41,9 +40,6 @@
 #endif
 #ifndef RSG_VELOCITY_WPE
 #define RSG_VELOCITY_WPE 0
-#endif
-#ifndef RSG_STRESS_EARLY_C
-#define RSG_STRESS_EARLY_C 1
 #endif
 #if RSG_STRESS_WPE > 0
 #define RSG_STRESS_ATTR __attribute__((amdgpu_waves_per_eu(RSG_STRESS_WPE)))
@@ -179,24 +175,97 @@ __device__ __forceinline__ void stage(T* lds, int max_u, const int32_t* __restri
   __syncthreads();
 }
 
-// LDS: gathers come from the block's staged copy through 16-bit local indices;
-// otherwise directly from HBM through the int32 connectivity.
-template <typename T, bool LDS, int U>
-__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_kernel(VelocityArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int b = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-  const int n0 = b * blockDim.x + threadIdx.x;
-  const bool live = n0 < a.n;
-  const int n = live ? n0 : a.n - 1;  // tail threads load a valid node and discard it
-  // streaming operands first (see stress_kernel)
+// Per-node operands of the velocity update that do not depend on a gather.
+template <typename T>
+struct VelIn {
   int c[8];
+  T idt, sx, sy, sz, v0, v1, v2;
+};
+
+template <typename T, bool LDS>
+__device__ __forceinline__ void velocity_load(const VelocityArgs<T>& a, int n, VelIn<T>& in) {
   if constexpr (LDS)
-    load_lconn(a.lconn, n, c);
+    load_lconn(a.lconn, n, in.c);
   else
-    load_conn(a.ncell, n, c);
-  T idt = a.idt[n];
-  const T sx = a.dnx[n], sy = a.dny[n], sz = a.dnz[n];
-  const T vold0 = a.V0[n], vold1 = a.V1[n], vold2 = a.V2[n];
+    load_conn(a.ncell, n, in.c);
+  in.idt = a.idt[n];
+  in.sx = a.dnx[n];
+  in.sy = a.dny[n];
+  in.sz = a.dnz[n];
+  in.v0 = a.V0[n];
+  in.v1 = a.V1[n];
+  in.v2 = a.V2[n];
+}
+
+// V += dt/rho * (div S + f) at node n, S gathered from S0..S5 (LDS or HBM).
+template <typename T>
+__device__ __forceinline__ void velocity_apply(const VelocityArgs<T>& a, int n, const VelIn<T>& in,
+                                               const T* S0, const T* S1, const T* S2, const T* S3,
+                                               const T* S4, const T* S5) {
+  const T sx = in.sx, sy = in.sy, sz = in.sz;
+  T f[8];
+  load8_cells(S0, in.c, f);
+  const T dx_sxx = dscale(diffsum<0>(f), sx);
+  load8_cells(S5, in.c, f);
+  const T dx_sxy = dscale(diffsum<0>(f), sx);
+  const T dy_sxy = dscale(diffsum<1>(f), sy);
+  load8_cells(S4, in.c, f);
+  const T dx_sxz = dscale(diffsum<0>(f), sx);
+  const T dz_sxz = dscale(diffsum<2>(f), sz);
+  load8_cells(S1, in.c, f);
+  const T dy_syy = dscale(diffsum<1>(f), sy);
+  load8_cells(S3, in.c, f);
+  const T dy_syz = dscale(diffsum<1>(f), sy);
+  const T dz_syz = dscale(diffsum<2>(f), sz);
+  load8_cells(S2, in.c, f);
+  const T dz_szz = dscale(diffsum<2>(f), sz);
+
+  T r0 = (dx_sxx + dy_sxy) + dz_sxz;
+  T r1 = (dx_sxy + dy_syy) + dz_syz;
+  T r2 = (dx_sxz + dy_syz) + dz_szz;
+
+  T idt = in.idt;
+  int sp = -1;
+  if (is_boxed(idt)) {
+    sp = unbox(idt);
+    idt = a.sp_idt[sp];
+    const int* off = a.sp_off + 4 * (size_t)sp;
+    double fsum[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double acc = 0.0;
+      for (int t = off[d]; t < off[d + 1]; t++) acc += a.amp[a.term[t]];
+      fsum[d] = acc;
+    }
+    r0 = r0 + (T)fsum[0];
+    r1 = r1 + (T)fsum[1];
+    r2 = r2 + (T)fsum[2];
+  }
+  const T v0 = in.v0 + idt * r0;
+  const T v1 = in.v1 + idt * r1;
+  const T v2 = in.v2 + idt * r2;
+  a.V0[n] = v0;
+  a.V1[n] = v1;
+  a.V2[n] = v2;
+  if (sp >= 0) {
+    const int slot = a.sp_sens[sp];
+    if (slot >= 0) {
+      a.trace[3 * slot + 0] = (double)v0;
+      a.trace[3 * slot + 1] = (double)v1;
+      a.trace[3 * slot + 2] = (double)v2;
+    }
+  }
+}
+
+// One workgroup's share of the velocity update: chunk b of the instance described by a
+// (CPT * blockDim.x nodes, thread t taking nodes t, t + blockDim.x, ...).  With LDS the
+// chunk's unique cells are staged first; the first node's streaming operands are
+// requested before the staging so their latency overlaps it.
+template <typename T, bool LDS, int U, int CPT>
+__device__ __forceinline__ void velocity_body(const VelocityArgs<T>& a, int b, unsigned char* smem) {
+  const int base = b * CPT * (int)blockDim.x + (int)threadIdx.x;
+  VelIn<T> in;
+  velocity_load<T, LDS>(a, base < a.n ? base : a.n - 1, in);  // tail threads load a valid node
   const T* S0 = a.S0;
   const T* S1 = a.S1;
   const T* S2 = a.S2;
@@ -214,58 +283,14 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_kern
     S4 = lds + 4 * a.max_u;
     S5 = lds + 5 * a.max_u;
   }
-  if (!live) return;
-
-  T f[8];
-  load8_cells(S0, c, f);
-  const T dx_sxx = dscale(diffsum<0>(f), sx);
-  load8_cells(S5, c, f);
-  const T dx_sxy = dscale(diffsum<0>(f), sx);
-  const T dy_sxy = dscale(diffsum<1>(f), sy);
-  load8_cells(S4, c, f);
-  const T dx_sxz = dscale(diffsum<0>(f), sx);
-  const T dz_sxz = dscale(diffsum<2>(f), sz);
-  load8_cells(S1, c, f);
-  const T dy_syy = dscale(diffsum<1>(f), sy);
-  load8_cells(S3, c, f);
-  const T dy_syz = dscale(diffsum<1>(f), sy);
-  const T dz_syz = dscale(diffsum<2>(f), sz);
-  load8_cells(S2, c, f);
-  const T dz_szz = dscale(diffsum<2>(f), sz);
-
-  T r0 = (dx_sxx + dy_sxy) + dz_sxz;
-  T r1 = (dx_sxy + dy_syy) + dz_syz;
-  T r2 = (dx_sxz + dy_syz) + dz_szz;
-
-  int sp = -1;
-  if (is_boxed(idt)) {
-    sp = unbox(idt);
-    idt = a.sp_idt[sp];
-    const int* off = a.sp_off + 4 * (size_t)sp;
-    double fsum[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      double acc = 0.0;
-      for (int t = off[d]; t < off[d + 1]; t++) acc += a.amp[a.term[t]];
-      fsum[d] = acc;
-    }
-    r0 = r0 + (T)fsum[0];
-    r1 = r1 + (T)fsum[1];
-    r2 = r2 + (T)fsum[2];
-  }
-  const T v0 = vold0 + idt * r0;
-  const T v1 = vold1 + idt * r1;
-  const T v2 = vold2 + idt * r2;
-  a.V0[n] = v0;
-  a.V1[n] = v1;
-  a.V2[n] = v2;
-  if (sp >= 0) {
-    const int slot = a.sp_sens[sp];
-    if (slot >= 0) {
-      a.trace[3 * slot + 0] = (double)v0;
-      a.trace[3 * slot + 1] = (double)v1;
-      a.trace[3 * slot + 2] = (double)v2;
-    }
+  if (base >= a.n) return;
+  velocity_apply(a, base, in, S0, S1, S2, S3, S4, S5);
+#pragma unroll 1
+  for (int j = 1; j < CPT; j++) {
+    const int n = base + j * (int)blockDim.x;
+    if (n >= a.n) break;
+    velocity_load<T, LDS>(a, n, in);
+    velocity_apply(a, n, in, S0, S1, S2, S3, S4, S5);
   }
 }
 
@@ -282,45 +307,41 @@ __host__ __device__ constexpr int tri(int i, int j) {
   return (i <= j) ? (i * 6 - i * (i - 1) / 2 + (j - i)) : (j * 6 - j * (j - 1) / 2 + (i - j));
 }
 
-template <typename T, bool LDS, int U>
-__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(StressArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int b = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-  const int e0 = b * blockDim.x + threadIdx.x;
-  const bool live = e0 < a.n;
-  const int e = live ? e0 : a.n - 1;  // tail threads load a valid cell and discard it
-  // Every operand that does not depend on a gather is requested first, so the whole
-  // cell's streaming traffic (links, spacings, 21 C, 6 S) is in flight at once and its
-  // latency overlaps the LDS staging below.
+// Per-cell operands of the stress update that do not depend on a gather: links,
+// spacings, the 21 C_IJ and the 6 stresses.
+template <typename T>
+struct StrIn {
   int c[8];
-  if constexpr (LDS)
-    load_lconn(a.lconn, e, c);
-  else
-    load_conn(a.cnode, e, c);
-  T sx = a.dcx[e];
-  const T sy = a.dcy[e], sz = a.dcz[e];
+  T sx, sy, sz;
   T cc[21];
-  if (RSG_STRESS_EARLY_C) {
+  T s[6];
+};
+
+template <typename T, bool LDS>
+__device__ __forceinline__ void stress_load(const StressArgs<T>& a, int e, StrIn<T>& in) {
+  if constexpr (LDS)
+    load_lconn(a.lconn, e, in.c);
+  else
+    load_conn(a.cnode, e, in.c);
+  in.sx = a.dcx[e];
+  in.sy = a.dcy[e];
+  in.sz = a.dcz[e];
 #pragma unroll
-    for (int j = 0; j < 21; j++) cc[j] = cload(a.C, a.ldc, j, e);
-  }
-  T s[6] = {a.S0[e], a.S1[e], a.S2[e], a.S3[e], a.S4[e], a.S5[e]};
-  const T* V0 = a.V0;
-  const T* V1 = a.V1;
-  const T* V2 = a.V2;
-  if constexpr (LDS) {
-    T* lds = reinterpret_cast<T*>(smem);
-    const T* const src[3] = {a.V0, a.V1, a.V2};
-    stage<T, 3, U>(lds, a.max_u, a.u_off, a.ulist, b, src);
-    V0 = lds;
-    V1 = lds + a.max_u;
-    V2 = lds + 2 * a.max_u;
-  }
-  if (!live) return;
-  if (!RSG_STRESS_EARLY_C) {
-#pragma unroll
-    for (int j = 0; j < 21; j++) cc[j] = cload(a.C, a.ldc, j, e);
-  }
+  for (int j = 0; j < 21; j++) in.cc[j] = cload(a.C, a.ldc, j, e);
+  in.s[0] = a.S0[e];
+  in.s[1] = a.S1[e];
+  in.s[2] = a.S2[e];
+  in.s[3] = a.S3[e];
+  in.s[4] = a.S4[e];
+  in.s[5] = a.S5[e];
+}
+
+// S += dt*(C : grad V + m) at cell e, V gathered from V0..V2 (LDS or HBM).
+template <typename T>
+__device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrIn<T>& in, const T* V0,
+                                             const T* V1, const T* V2) {
+  T sx = in.sx;
+  const T sy = in.sy, sz = in.sz;
   int sp = -1;
   if (is_boxed(sx)) {
     sp = unbox(sx);
@@ -328,15 +349,15 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(S
   }
   // delV[i][k] = D_i V_k (i: derivative axis, k: velocity component)
   T f[8];
-  load8_nodes(V0, c, f);
+  load8_nodes(V0, in.c, f);
   const T d00 = dscale(diffsum<0>(f), sx);
   const T d10 = dscale(diffsum<1>(f), sy);
   const T d20 = dscale(diffsum<2>(f), sz);
-  load8_nodes(V1, c, f);
+  load8_nodes(V1, in.c, f);
   const T d01 = dscale(diffsum<0>(f), sx);
   const T d11 = dscale(diffsum<1>(f), sy);
   const T d21 = dscale(diffsum<2>(f), sz);
-  load8_nodes(V2, c, f);
+  load8_nodes(V2, in.c, f);
   const T d02 = dscale(diffsum<0>(f), sx);
   const T d12 = dscale(diffsum<1>(f), sy);
   const T d22 = dscale(diffsum<2>(f), sz);
@@ -352,7 +373,8 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(S
       m[k] = acc;
     }
   }
-
+  const T* cc = in.cc;
+  T* s = in.s;
 #pragma unroll
   for (int ks = 0; ks < 6; ks++) {
     T acc = cc[tri(ks, 0)] * d00 + cc[tri(ks, 1)] * d11 + cc[tri(ks, 2)] * d22 +
@@ -371,6 +393,93 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(S
   a.S3[e] = s[3];
   a.S4[e] = s[4];
   a.S5[e] = s[5];
+}
+
+// One workgroup's share of the stress update: chunk b of the instance described by a.
+// Every operand that does not depend on a gather is requested first, so a cell's whole
+// streaming traffic (links, spacings, 21 C, 6 S) is in flight at once and its latency
+// overlaps the LDS staging of the chunk's unique nodes.
+template <typename T, bool LDS, int U, int CPT>
+__device__ __forceinline__ void stress_body(const StressArgs<T>& a, int b, unsigned char* smem) {
+  const int base = b * CPT * (int)blockDim.x + (int)threadIdx.x;
+  StrIn<T> in;
+  stress_load<T, LDS>(a, base < a.n ? base : a.n - 1, in);  // tail threads load a valid cell
+  const T* V0 = a.V0;
+  const T* V1 = a.V1;
+  const T* V2 = a.V2;
+  if constexpr (LDS) {
+    T* lds = reinterpret_cast<T*>(smem);
+    const T* const src[3] = {a.V0, a.V1, a.V2};
+    stage<T, 3, U>(lds, a.max_u, a.u_off, a.ulist, b, src);
+    V0 = lds;
+    V1 = lds + a.max_u;
+    V2 = lds + 2 * a.max_u;
+  }
+  if (base >= a.n) return;
+  stress_apply(a, base, in, V0, V1, V2);
+#pragma unroll 1
+  for (int j = 1; j < CPT; j++) {
+    const int e = base + j * (int)blockDim.x;
+    if (e >= a.n) break;
+    stress_load<T, LDS>(a, e, in);
+    stress_apply(a, e, in, V0, V1, V2);
+  }
+}
+
+// Single-instance launches: one workgroup per block, XCD-contiguous block order.
+template <typename T, bool LDS, int U, int CPT>
+__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_kernel(VelocityArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  velocity_body<T, LDS, U, CPT>(a, a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x, smem);
+}
+
+template <typename T, bool LDS, int U, int CPT>
+__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(StressArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  stress_body<T, LDS, U, CPT>(a, a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x, smem);
+}
+
+// Batched launches: a persistent grid walks the work table (work[0] items of (instance,
+// block) after a two-int header) and reads each instance's argument block from desc.
+// Step k of the current K-step block runs only for instances with k < kmax, so instances
+// of different lengths stop exactly at their own step count.  The table and the
+// descriptors live in device memory, so a captured graph of these launches stays valid
+// while instances join and leave.
+template <typename T, bool LDS, int U, int CPT>
+__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_batch_kernel(
+    const BatchDesc<T>* __restrict__ desc, const int32_t* __restrict__ work, int k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n_work = work[0];
+  const int2* items = reinterpret_cast<const int2*>(work + 2);
+  for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
+    const int2 it = items[w];
+    const BatchDesc<T>* d = desc + it.x;
+    if (k < d->kmax) {
+      VelocityArgs<T> a = d->v;
+      a.amp = d->amp_base + (size_t)k * d->nch;
+      a.trace = d->trace_base ? d->trace_base + (size_t)(d->trace_row0 + k) * d->row : nullptr;
+      velocity_body<T, LDS, U, CPT>(a, it.y, smem);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, bool LDS, int U, int CPT>
+__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_batch_kernel(
+    const BatchDesc<T>* __restrict__ desc, const int32_t* __restrict__ work, int k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n_work = work[0];
+  const int2* items = reinterpret_cast<const int2*>(work + 2);
+  for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
+    const int2 it = items[w];
+    const BatchDesc<T>* d = desc + it.x;
+    if (k < d->kmax) {
+      StressArgs<T> a = d->s;
+      a.amp = d->amp_base + (size_t)k * d->nch;
+      stress_body<T, LDS, U, CPT>(a, it.y, smem);
+    }
+    __syncthreads();
+  }
 }
 
 // Energy: one FP64 partial per block for kinetic and stress-norm sums.
@@ -410,48 +519,51 @@ __global__ __launch_bounds__(256) void energy_kernel(EnergyArgs<T> a) {
 template <typename Args, typename K>
 hipError_t launch1d(K kernel, const Args& a, int n, int block, size_t lds, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (block < 64 || block > RSG_MAX_BLOCK || (block % 64) != 0) return hipErrorInvalidValue;
-  const int grid = (n + block - 1) / block;
+  if (block < 64 || block > RSG_MAX_BLOCK || (block % 64) != 0 || a.cpt < 1) return hipErrorInvalidValue;
+  const int chunk = block * a.cpt;
+  const int grid = (n + chunk - 1) / chunk;
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, s, a);
   return hipGetLastError();
 }
 
 // Picks the kernel instance: direct gathers, or LDS staging with U = ceil(max_u/block)
-// entries per thread (general loop beyond 4).
-template <typename T, typename Args, typename F>
-hipError_t launch_gather(const Args& a, int block, int ncomp, hipStream_t s, F kernel_for) {
-  if (!a.lconn) return launch1d(kernel_for(std::integral_constant<int, -1>()), a, a.n, block, 0, s);
-  const size_t lds = (size_t)ncomp * a.max_u * sizeof(T);
-  const int u = (a.max_u + block - 1) / block;
-  switch (u) {
-    case 0:
-    case 1: return launch1d(kernel_for(std::integral_constant<int, 1>()), a, a.n, block, lds, s);
-    case 2: return launch1d(kernel_for(std::integral_constant<int, 2>()), a, a.n, block, lds, s);
-    case 3: return launch1d(kernel_for(std::integral_constant<int, 3>()), a, a.n, block, lds, s);
-    case 4: return launch1d(kernel_for(std::integral_constant<int, 4>()), a, a.n, block, lds, s);
-    default: return launch1d(kernel_for(std::integral_constant<int, 0>()), a, a.n, block, lds, s);
+// entries per thread (general loop beyond 4), for 1, 2 or 4 entities per thread.
+template <typename F>
+hipError_t with_variant(bool lds, int max_u, int block, int cpt, F&& f) {
+  auto by_u = [&](auto cptc) -> hipError_t {
+    if (!lds) return f(std::false_type(), std::integral_constant<int, 0>(), cptc);
+    switch ((max_u + block - 1) / block) {
+      case 0:
+      case 1: return f(std::true_type(), std::integral_constant<int, 1>(), cptc);
+      case 2: return f(std::true_type(), std::integral_constant<int, 2>(), cptc);
+      case 3: return f(std::true_type(), std::integral_constant<int, 3>(), cptc);
+      case 4: return f(std::true_type(), std::integral_constant<int, 4>(), cptc);
+      default: return f(std::true_type(), std::integral_constant<int, 0>(), cptc);
+    }
+  };
+  switch (cpt) {
+    case 1: return by_u(std::integral_constant<int, 1>());
+    case 2: return by_u(std::integral_constant<int, 2>());
+    case 4: return by_u(std::integral_constant<int, 4>());
+    default: return hipErrorInvalidValue;
   }
 }
 
 template <typename T>
 hipError_t launch_velocity(const VelocityArgs<T>& a, int block, hipStream_t s) {
-  return launch_gather<T>(a, block, 6, s, [](auto u) {
-    constexpr int U = decltype(u)::value;
-    if constexpr (U < 0)
-      return velocity_kernel<T, false, 0>;
-    else
-      return velocity_kernel<T, true, U>;
+  const bool lds = a.lconn != nullptr;
+  return with_variant(lds, a.max_u, block, a.cpt, [&](auto L, auto U, auto C) {
+    return launch1d(velocity_kernel<T, decltype(L)::value, decltype(U)::value, decltype(C)::value>, a, a.n,
+                    block, lds ? (size_t)6 * a.max_u * sizeof(T) : 0, s);
   });
 }
 
 template <typename T>
 hipError_t launch_stress(const StressArgs<T>& a, int block, hipStream_t s) {
-  return launch_gather<T>(a, block, 3, s, [](auto u) {
-    constexpr int U = decltype(u)::value;
-    if constexpr (U < 0)
-      return stress_kernel<T, false, 0>;
-    else
-      return stress_kernel<T, true, U>;
+  const bool lds = a.lconn != nullptr;
+  return with_variant(lds, a.max_u, block, a.cpt, [&](auto L, auto U, auto C) {
+    return launch1d(stress_kernel<T, decltype(L)::value, decltype(U)::value, decltype(C)::value>, a, a.n, block,
+                    lds ? (size_t)3 * a.max_u * sizeof(T) : 0, s);
   });
 }
 
@@ -468,6 +580,31 @@ hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t
 }
 hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t s) {
   return launch_stress<float>(a, block, s);
+}
+template <typename T>
+hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* work, int k, int grid, int block, bool lds,
+                        int max_u, int cpt, bool velocity, hipStream_t s) {
+  if (grid <= 0) return hipSuccess;
+  if (block < 64 || block > RSG_MAX_BLOCK || (block % 64) != 0) return hipErrorInvalidValue;
+  const size_t bytes = lds ? (size_t)(velocity ? 6 : 3) * max_u * sizeof(T) : 0;
+  return with_variant(lds, max_u, block, cpt, [&](auto L, auto U, auto C) {
+    constexpr bool l = decltype(L)::value;
+    constexpr int u = decltype(U)::value, c = decltype(C)::value;
+    if (velocity)
+      hipLaunchKernelGGL((velocity_batch_kernel<T, l, u, c>), dim3(grid), dim3(block), bytes, s, desc, work, k);
+    else
+      hipLaunchKernelGGL((stress_batch_kernel<T, l, u, c>), dim3(grid), dim3(block), bytes, s, desc, work, k);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* work, int k, int grid, int block,
+                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s) {
+  return launch_batch<double>(d, work, k, grid, block, lds, max_u, cpt, velocity, s);
+}
+hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* work, int k, int grid, int block,
+                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s) {
+  return launch_batch<float>(d, work, k, grid, block, lds, max_u, cpt, velocity, s);
 }
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(energy_kernel<double>, dim3(nblocks), dim3(256), 0, s, a);

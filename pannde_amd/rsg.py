@@ -32,7 +32,9 @@ STATE_S = ("Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy")
 EXPORTS = ("rsg_create", "rsg_set_channels", "rsg_set_sensors", "rsg_step", "rsg_get_state",
            "rsg_set_state", "rsg_get_traces", "rsg_synchronize", "rsg_get_time", "rsg_energy", "rsg_set_timing",
            "rsg_get_kernel_times", "rsg_device_bytes", "rsg_destroy", "rsg_last_error",
-           "rsg_disc_support", "rsg_signal_eval")
+           "rsg_disc_support", "rsg_signal_eval", "rsg_create_ex", "rsg_get_tuning", "rsg_batch_create",
+           "rsg_batch_add", "rsg_batch_remove", "rsg_batch_run", "rsg_batch_synchronize", "rsg_batch_slot_status",
+           "rsg_batch_destroy")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -55,6 +57,26 @@ class ModelDesc(ctypes.Structure):
 class ChannelDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("component", ctypes.c_int32), ("n_support", ctypes.c_int32),
                 ("support", _ip), ("n_signal", ctypes.c_int32), ("signal_t", _dp), ("signal_v", _dp)]
+
+
+class TuningDesc(ctypes.Structure):
+    _fields_ = [("block_velocity", ctypes.c_int32), ("block_stress", ctypes.c_int32),
+                ("per_thread_velocity", ctypes.c_int32), ("per_thread_stress", ctypes.c_int32),
+                ("lds", ctypes.c_int32), ("tile", ctypes.c_int32 * 3), ("c_layout", ctypes.c_int32),
+                ("xcd_remap", ctypes.c_int32)]
+
+
+def tuning_desc(tuning):
+    """dict -> TuningDesc (missing keys take the library defaults)."""
+    t = TuningDesc()
+    if tuning:
+        for k, v in tuning.items():
+            if k == "tile":
+                for d in range(3):
+                    t.tile[d] = int(v[d])
+            else:
+                setattr(t, k, int(v))
+    return t
 
 
 class RSGError(RuntimeError):
@@ -93,6 +115,18 @@ def lib():
         L.rsg_disc_support.argtypes = [ctypes.POINTER(MeshDesc), ctypes.c_int32, _dp, ctypes.c_double,
                                        ctypes.c_double, _ip, _ip]
         L.rsg_signal_eval.argtypes = [ctypes.c_int32, _dp, _dp, ctypes.c_int32, _dp, _dp]
+        L.rsg_create_ex.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(ModelDesc), ctypes.c_int32,
+                                    ctypes.POINTER(TuningDesc), ctypes.POINTER(vp)]
+        L.rsg_get_tuning.argtypes = [vp, ctypes.POINTER(TuningDesc)]
+        L.rsg_batch_create.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.POINTER(vp)]
+        L.rsg_batch_add.argtypes = [vp, vp, ctypes.c_int64, _ip]
+        L.rsg_batch_remove.argtypes = [vp, ctypes.c_int32]
+        L.rsg_batch_run.argtypes = [vp, ctypes.c_int32, _ip]
+        L.rsg_batch_synchronize.argtypes = [vp]
+        L.rsg_batch_slot_status.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
+                                            ctypes.POINTER(ctypes.c_int64)]
+        L.rsg_batch_destroy.argtypes = [vp]
         for name in EXPORTS:
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
@@ -162,7 +196,7 @@ class DeviceSolver:
     reference's row order; ``precision`` is RSG_FP64 (reference arithmetic) or RSG_FP32."""
 
     def __init__(self, x, y, z, cells, rho, cij, dt, force=(), moment=(), moment_form=0,
-                 sensors=(), precision=RSG_FP64):
+                 sensors=(), precision=RSG_FP64, tuning=None):
         L = lib()
         self._mesh = _Mesh(x, y, z, cells)
         self.n_nodes = self._mesh.desc.n_nodes
@@ -182,7 +216,9 @@ class DeviceSolver:
         for k in range(21):
             md.cij[k] = fd(cij[k])
         h = ctypes.c_void_p()
-        _check(L.rsg_create(ctypes.byref(self._mesh.desc), ctypes.byref(md), int(precision), ctypes.byref(h)))
+        td = tuning_desc(tuning)
+        _check(L.rsg_create_ex(ctypes.byref(self._mesh.desc), ctypes.byref(md), int(precision), ctypes.byref(td),
+                               ctypes.byref(h)))
         self._h = h
         self._keep = []
         self.dt = float(dt)
@@ -190,9 +226,16 @@ class DeviceSolver:
         self.set_sensors(sensors)
 
     @classmethod
-    def from_case(cls, case: "_cases.Case", precision=RSG_FP64) -> "DeviceSolver":
+    def from_case(cls, case: "_cases.Case", precision=RSG_FP64, tuning=None) -> "DeviceSolver":
         return cls(case.x, case.y, case.z, case.cells, case.rho, case.cij, case.dt, case.force, case.moment,
-                   case.moment_form, case.sensors, precision)
+                   case.moment_form, case.sensors, precision, tuning)
+
+    def tuning(self):
+        t = TuningDesc()
+        _check(lib().rsg_get_tuning(self._h, ctypes.byref(t)))
+        return dict(block_velocity=t.block_velocity, block_stress=t.block_stress,
+                    per_thread_velocity=t.per_thread_velocity, per_thread_stress=t.per_thread_stress,
+                    lds=t.lds, tile=tuple(t.tile), c_layout=t.c_layout, xcd_remap=t.xcd_remap)
 
     # -- sources / sensors
     def set_sources(self, force=(), moment=()):
@@ -295,6 +338,52 @@ class DeviceSolver:
     def close(self):
         if getattr(self, "_h", None):
             lib().rsg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many DeviceSolver instances advanced together (rsg_batch_*): one persistent launch
+    per kernel per step over all of them, K-step blocks replayed from a HIP graph."""
+
+    def __init__(self, precision=RSG_FP64, steps_per_block=50, grid=0, use_graph=True):
+        h = ctypes.c_void_p()
+        _check(lib().rsg_batch_create(int(precision), int(steps_per_block), int(grid), 1 if use_graph else 0,
+                                      ctypes.byref(h)))
+        self._h = h
+        self.members = {}
+
+    def add(self, solver: DeviceSolver, nsteps: int) -> int:
+        slot = ctypes.c_int32(0)
+        _check(lib().rsg_batch_add(self._h, solver._h, int(nsteps), ctypes.byref(slot)))
+        self.members[slot.value] = solver
+        return slot.value
+
+    def remove(self, slot: int) -> DeviceSolver:
+        _check(lib().rsg_batch_remove(self._h, int(slot)))
+        return self.members.pop(slot)
+
+    def run(self, nblocks=1) -> int:
+        left = ctypes.c_int32(0)
+        _check(lib().rsg_batch_run(self._h, int(nblocks), ctypes.byref(left)))
+        return left.value
+
+    def synchronize(self):
+        _check(lib().rsg_batch_synchronize(self._h))
+
+    def status(self, slot):
+        d, t = ctypes.c_int64(0), ctypes.c_int64(0)
+        _check(lib().rsg_batch_slot_status(self._h, int(slot), ctypes.byref(d), ctypes.byref(t)))
+        return d.value, t.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rsg_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

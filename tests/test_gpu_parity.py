@@ -58,19 +58,37 @@ def test_device_matches_reference_fixture(name):
     assert_same(run_device(case), (ref["V"], ref["S"], ref["time"], ref["traces"]))
 
 
-@pytest.mark.parametrize("block", [64, 128, 256, 512])
-def test_block_size_does_not_change_results(block, monkeypatch):
-    monkeypatch.setenv("RSG_BLOCK_VELOCITY", str(block))
-    monkeypatch.setenv("RSG_BLOCK_STRESS", str(block))
-    case, ref = load_golden("cfrp_holes_rcm")
-    assert_same(run_device(case), (ref["V"], ref["S"], ref["time"], ref["traces"]))
+TUNINGS = [
+    dict(block_velocity=64, block_stress=64),
+    dict(block_velocity=128, block_stress=512),
+    dict(block_velocity=512, block_stress=128),
+    dict(per_thread_velocity=2, per_thread_stress=2),
+    dict(per_thread_velocity=4, per_thread_stress=4, block_velocity=128, block_stress=128),
+    dict(lds=-1),
+    dict(lds=-1, per_thread_velocity=2, per_thread_stress=4),
+    dict(tile=(1 << 20, 1, 1)),
+    dict(tile=(16, 4, 4), c_layout=-1, xcd_remap=-1),
+]
 
 
-def test_unsupported_block_size_is_rejected(monkeypatch):
-    monkeypatch.setenv("RSG_BLOCK_STRESS", "1024")
+@pytest.mark.parametrize("tuning", TUNINGS, ids=[str(sorted(t.items())) for t in TUNINGS])
+@pytest.mark.parametrize("name", ["cfrp_holes_rcm", "water_column_moment"])
+def test_tuning_does_not_change_results(name, tuning):
+    case, ref = load_golden(name)
+    s = rsg.DeviceSolver.from_case(case, tuning=tuning)
+    try:
+        t = s.solve(case.nsteps)
+        V, S = s.get_state()
+        assert_same((V, S, t, s.traces()), (ref["V"], ref["S"], ref["time"], ref["traces"]))
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("tuning", [dict(block_stress=1024), dict(block_velocity=96), dict(per_thread_stress=3)])
+def test_unsupported_tuning_is_rejected(tuning):
     case, _ = load_golden("al_block_cellrho")
-    with pytest.raises(rsg.RSGError, match="workgroup size"):
-        rsg.DeviceSolver.from_case(case)
+    with pytest.raises(rsg.RSGError, match="workgroup size|per thread"):
+        rsg.DeviceSolver.from_case(case, tuning=tuning)
 
 
 @pytest.mark.parametrize("seed", [201, 202, 203, 204])
