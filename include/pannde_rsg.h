@@ -120,7 +120,8 @@ typedef struct {
   int32_t block_stress;        /* workgroup size of the stress kernel (256) */
   int32_t per_thread_velocity; /* nodes per thread: 1, 2 or 4 (1) */
   int32_t per_thread_stress;   /* cells per thread: 1, 2 or 4 (1) */
-  int32_t lds;                 /* 1: stage each workgroup's gathers in LDS, -1: gather from HBM (1) */
+  int32_t lds_velocity;        /* 1: stage each workgroup's gathers in LDS, -1: gather from HBM (1) */
+  int32_t lds_stress;          /* the same for the stress kernel (1) */
   int32_t tile[3];             /* locality brick in cells along x, y, z (8, 8, 4) */
   int32_t c_layout;            /* 1: C in 64-cell blocks [n/64][21][64], -1: [21][n] (1) */
   int32_t xcd_remap;           /* 1: XCD-contiguous workgroup order, -1: hardware order (1) */
@@ -204,7 +205,8 @@ int rsg_last_error(char* buf, size_t len);
  * state and traces with rsg_get_state / rsg_get_traces after rsg_batch_synchronize. */
 typedef struct rsg_batch_s* rsg_batch;
 
-/* steps_per_block: K (0: 50).  grid: workgroups of the persistent launches (0: 8 per CU).
+/* steps_per_block: K (0: 50).  grid: workgroups of the persistent launches (0: 8 per CU;
+ * -1: one workgroup per work item, the grid sized to the work table's capacity).
  * use_graph: 0 launches every step directly instead of replaying a captured graph. */
 int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, int32_t use_graph,
                      rsg_batch* out);

@@ -1,0 +1,56 @@
+"""Autotuner for the RSG kernels.
+
+Sweeps, for each of the two kernels independently, the workgroup size {64, 128, 256, 512},
+the entities per thread {1, 2, 4} and LDS staging on/off, timing each candidate with the
+library's HIP-event kernel timers on a representative job, and returns the fastest
+choice as an ``rsg_tuning`` dict.  Every candidate computes bit-identical results, so the
+choice only affects speed.  Handles of one batch must share workgroup sizes and
+per-thread counts; tune once on a representative instance and create every member of the
+batch with the result.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+from typing import Dict, List, Optional, Tuple
+
+from . import rsg
+
+BLOCKS = (64, 128, 256, 512)
+PER_THREAD = (1, 2, 4)
+LDS = (1, -1)
+
+
+def _time_candidate(case, precision, block, cpt, lds, steps, warmup):
+    tuning = dict(block_velocity=block, block_stress=block, per_thread_velocity=cpt, per_thread_stress=cpt,
+                  lds_velocity=lds, lds_stress=lds)
+    s = rsg.DeviceSolver.from_case(case, precision, tuning=tuning)
+    try:
+        s.solve(warmup)
+        s.synchronize()
+        s.kernel_times()
+        s.set_timing(True)
+        s.solve(steps)
+        s.synchronize()
+        vms, sms, n = s.kernel_times()
+        return vms / n, sms / n
+    finally:
+        s.close()
+
+
+def tune(case, precision=rsg.RSG_FP32, steps=20, warmup=3, blocks=BLOCKS, per_thread=PER_THREAD, lds=LDS,
+         log=None) -> Tuple[Dict, List[Dict]]:
+    """Returns (best tuning dict, list of measured candidates)."""
+    results = []
+    for block, cpt, l in itertools.product(blocks, per_thread, lds):
+        t0 = time.perf_counter()
+        vms, sms = _time_candidate(case, precision, block, cpt, l, steps, warmup)
+        results.append(dict(block=block, per_thread=cpt, lds=l, velocity_ms=vms, stress_ms=sms))
+        if log:
+            log("autotune: block %3d x %d per thread, lds %2d: velocity %.4f ms, stress %.4f ms (%.1f s)"
+                % (block, cpt, l, vms, sms, time.perf_counter() - t0))
+    bv = min(results, key=lambda r: r["velocity_ms"])
+    bs = min(results, key=lambda r: r["stress_ms"])
+    best = dict(block_velocity=bv["block"], per_thread_velocity=bv["per_thread"], lds_velocity=bv["lds"],
+                block_stress=bs["block"], per_thread_stress=bs["per_thread"], lds_stress=bs["lds"])
+    return best, results

@@ -304,3 +304,40 @@ def water_column_block(seed=3, nx=12, ny=10, nz=6, dx=1.0e-4, nsteps=40):
     case.moment_form = 1
     case.name = "water_column_%dx%dx%d" % (nx, ny, nz)
     return case
+
+
+TI64 = dict(rho=4430.0, lam=90.0e9, mu=44.0e9)
+AL = dict(rho=2700.0, lam=51.0e9, mu=26.0e9)
+
+
+def batch_instances(seed=4, count=64, min_cells=32 ** 3, max_cells=128 ** 3, nsteps=1500, dx=1.0e-4,
+                    n_sensors=8):
+    """Independent parametric instances for one batched launch: cell counts log-uniform in
+    [min_cells, max_cells] with seeded box shapes, isotropic Al or Ti-6Al-4V with a seeded
+    +-5 % perturbation of rho, lambda and mu, a 5 MHz 3-cycle burst on a disc at a seeded
+    position of the top face, each with its own CFL dt."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        target = math.exp(rng.uniform(math.log(min_cells), math.log(max_cells)))
+        a = np.exp(rng.uniform(-0.5, 0.5, size=3))
+        a /= a.prod() ** (1 / 3)
+        n = np.maximum(8, np.round(a * target ** (1 / 3))).astype(int)
+        nx, ny, nz = (int(v) for v in n)
+        base = AL if rng.uniform() < 0.5 else TI64
+        rho, lam, mu = (base[k] * (1 + rng.uniform(-0.05, 0.05)) for k in ("rho", "lam", "mu"))
+        x, y, z, cells, _ = structured_mesh(nx, ny, nz, dx, dx, dx)
+        nc = cells.shape[0]
+        dt = cfl_dt(math.sqrt((lam + 2 * mu) / rho), dx, dx, dx)
+        t, v = tone_burst(5.0e6, 3, dt)
+        r = min(2.0e-3, 0.3 * min(nx, ny) * dx)
+        cx = rng.uniform(r, nx * dx - r)
+        cy = rng.uniform(r, ny * dx - r)
+        disc = Disc(2, (cx, cy, nz * dx), r, cell0_dz(x, y, z, cells), t, v)
+        out.append(Case(name="batch%02d_%dx%dx%d" % (i, nx, ny, nz), x=x, y=y, z=z, cells=cells,
+                        rho=Field(FIELD_CELL, np.full(nc, rho)),
+                        cij=[Field(FIELD_CONSTANT, np.array([c])) for c in
+                             (isotropic_c(lam, mu)[i_, j_] for i_, j_ in CIJ_PAIRS)],
+                        dt=dt, nsteps=nsteps, force=[disc],
+                        sensors=seeded_sensors(rng, top_face_nodes(z), n_sensors), seed=seed))
+    return out

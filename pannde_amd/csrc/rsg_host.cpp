@@ -131,7 +131,7 @@ struct rsg_instance {
   bool in_batch = false;          // stepped by a batch, not by rsg_step
   hipStream_t batch_stream = nullptr;  // the batch's stream while in_batch
   int xcd = 1;                    // XCD-contiguous workgroup remap
-  int lds = 1;                    // stage gathers through LDS when the block sets fit
+  int lds_v = 1, lds_s = 1;       // stage gathers through LDS when the block sets fit
   int c_blocked = 1;              // C as [n/64][21][64] instead of [21][n]
   int tile[3] = {8, 8, 4};        // locality brick (x, y, z) in cells
 
@@ -660,7 +660,8 @@ int rsg_get_tuning(rsg_handle h, rsg_tuning* t) {
   t->block_stress = h->block_s;
   t->per_thread_velocity = h->cpt_v;
   t->per_thread_stress = h->cpt_s;
-  t->lds = h->lds ? 1 : -1;
+  t->lds_velocity = h->lds_v ? 1 : -1;
+  t->lds_stress = h->lds_s ? 1 : -1;
   for (int d = 0; d < 3; d++) t->tile[d] = h->tile[d];
   t->c_layout = h->c_blocked ? 1 : -1;
   t->xcd_remap = h->xcd ? 1 : -1;
@@ -695,7 +696,8 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if (t.block_stress) h->block_s = t.block_stress;
     if (t.per_thread_velocity) h->cpt_v = t.per_thread_velocity;
     if (t.per_thread_stress) h->cpt_s = t.per_thread_stress;
-    if (t.lds) h->lds = t.lds > 0;
+    if (t.lds_velocity) h->lds_v = t.lds_velocity > 0;
+    if (t.lds_stress) h->lds_s = t.lds_stress > 0;
     if (t.c_layout) h->c_blocked = t.c_layout > 0;
     if (t.xcd_remap) h->xcd = t.xcd_remap > 0;
     for (int d = 0; d < 3; d++) {
@@ -819,7 +821,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->ncell = (int32_t*)p;
     LdsPlan plan;
-    if (h->lds && build_lds_plan(conn, nn, h->block_v * h->cpt_v, plan) &&
+    if (h->lds_v && build_lds_plan(conn, nn, h->block_v * h->cpt_v, plan) &&
         (size_t)6 * plan.max_u * h->elt() <= lds_limit) {
       if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
       h->v_uoff = (int32_t*)p;
@@ -860,7 +862,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->cnode = (int32_t*)p;
     LdsPlan plan;
-    if (h->lds && build_lds_plan(conn, nc, h->block_s * h->cpt_s, plan) &&
+    if (h->lds_s && build_lds_plan(conn, nc, h->block_s * h->cpt_s, plan) &&
         (size_t)3 * plan.max_u * h->elt() <= lds_limit) {
       if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
       h->s_uoff = (int32_t*)p;
@@ -1210,7 +1212,8 @@ int rsg_signal_eval(int32_t n, const double* t, const double* v, int32_t n_eval,
 struct rsg_batch_s {
   int32_t precision = RSG_FP64;
   int K = 50;
-  int grid = 0;
+  int grid = 0;         // workgroups of the persistent launches
+  bool dense = false;   // one workgroup per work item (grid = table capacity)
   bool use_graph = true;
   hipStream_t stream = nullptr;
   struct Slot {
@@ -1306,20 +1309,22 @@ int batch_launch_block(rsg_batch_s* b) {
   const auto* desc = (const rsg::BatchDesc<T>*)b->ctl_dev;
   const int32_t* wv = (const int32_t*)(b->ctl_dev + b->off_wv);
   const int32_t* ws = (const int32_t*)(b->ctl_dev + b->off_ws);
+  const int gv = b->dense ? (int)b->cap_wv : b->grid;
+  const int gs = b->dense ? (int)b->cap_ws : b->grid;
   auto enqueue = [&]() -> hipError_t {
     for (int k = 0; k < b->K; k++) {
       hipError_t e;
       if (sizeof(T) == 8) {
-        e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv, k, b->grid, b->block_v, b->lds_v,
+        e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv, k, gv, b->block_v, b->lds_v,
                                   b->max_u_v, b->cpt_v, true, b->stream);
         if (e != hipSuccess) return e;
-        e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws, k, b->grid, b->block_s, b->lds_s,
+        e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws, k, gs, b->block_s, b->lds_s,
                                   b->max_u_s, b->cpt_s, false, b->stream);
       } else {
-        e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv, k, b->grid, b->block_v, b->lds_v,
+        e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv, k, gv, b->block_v, b->lds_v,
                                   b->max_u_v, b->cpt_v, true, b->stream);
         if (e != hipSuccess) return e;
-        e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws, k, b->grid, b->block_s, b->lds_s,
+        e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws, k, gs, b->block_s, b->lds_s,
                                   b->max_u_s, b->cpt_s, false, b->stream);
       }
       if (e != hipSuccess) return e;
@@ -1459,14 +1464,16 @@ int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, i
   *out = nullptr;
   if (precision != RSG_FP64 && precision != RSG_FP32) return fail(RSG_ERR_ARG, "bad precision %d", precision);
   if (steps_per_block < 0 || steps_per_block > 4096) return fail(RSG_ERR_ARG, "steps_per_block %d", steps_per_block);
-  if (grid < 0) return fail(RSG_ERR_ARG, "grid %d", grid);
+  if (grid < -1) return fail(RSG_ERR_ARG, "grid %d", grid);
   std::unique_ptr<rsg_batch_s> b(new (std::nothrow) rsg_batch_s());
   if (!b) return fail(RSG_ERR_NOMEM, "host allocation failed");
   b->precision = precision;
   b->K = steps_per_block ? steps_per_block : 50;
   b->use_graph = use_graph != 0;
-  if (grid) {
+  if (grid > 0) {
     b->grid = grid;
+  } else if (grid < 0) {
+    b->dense = true;
   } else {
     int dev = 0, cus = 0;
     HIP_TRY(hipGetDevice(&dev));

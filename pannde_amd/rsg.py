@@ -62,7 +62,8 @@ class ChannelDesc(ctypes.Structure):
 class TuningDesc(ctypes.Structure):
     _fields_ = [("block_velocity", ctypes.c_int32), ("block_stress", ctypes.c_int32),
                 ("per_thread_velocity", ctypes.c_int32), ("per_thread_stress", ctypes.c_int32),
-                ("lds", ctypes.c_int32), ("tile", ctypes.c_int32 * 3), ("c_layout", ctypes.c_int32),
+                ("lds_velocity", ctypes.c_int32), ("lds_stress", ctypes.c_int32), ("tile", ctypes.c_int32 * 3),
+                ("c_layout", ctypes.c_int32),
                 ("xcd_remap", ctypes.c_int32)]
 
 
@@ -235,7 +236,8 @@ class DeviceSolver:
         _check(lib().rsg_get_tuning(self._h, ctypes.byref(t)))
         return dict(block_velocity=t.block_velocity, block_stress=t.block_stress,
                     per_thread_velocity=t.per_thread_velocity, per_thread_stress=t.per_thread_stress,
-                    lds=t.lds, tile=tuple(t.tile), c_layout=t.c_layout, xcd_remap=t.xcd_remap)
+                    lds_velocity=t.lds_velocity, lds_stress=t.lds_stress, tile=tuple(t.tile),
+                    c_layout=t.c_layout, xcd_remap=t.xcd_remap)
 
     # -- sources / sensors
     def set_sources(self, force=(), moment=()):

@@ -64,8 +64,9 @@ TUNINGS = [
     dict(block_velocity=512, block_stress=128),
     dict(per_thread_velocity=2, per_thread_stress=2),
     dict(per_thread_velocity=4, per_thread_stress=4, block_velocity=128, block_stress=128),
-    dict(lds=-1),
-    dict(lds=-1, per_thread_velocity=2, per_thread_stress=4),
+    dict(lds_velocity=-1, lds_stress=-1),
+    dict(lds_velocity=-1, lds_stress=1, per_thread_stress=2),
+    dict(lds_velocity=1, lds_stress=-1, per_thread_velocity=2, per_thread_stress=4),
     dict(tile=(1 << 20, 1, 1)),
     dict(tile=(16, 4, 4), c_layout=-1, xcd_remap=-1),
 ]
