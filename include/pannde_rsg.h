@@ -193,10 +193,13 @@ int rsg_destroy(rsg_handle h);
 int rsg_last_error(char* buf, size_t len);
 
 /* ---------------------------------------------------------------- batches
- * Many independent instances advanced together: one persistent launch per kernel per
- * step covers every instance of the batch (a device work table of (instance, block)
- * items and a device table of per-instance arguments), and K consecutive steps are
- * captured once in a HIP graph.  Instances join and leave between K-step blocks; each
+ * Many independent instances advanced together: one launch per kernel per step covers
+ * every instance of a group (a device work table of (instance, block) items and a device
+ * table of per-instance arguments), and the K consecutive steps of a block are captured
+ * once in a HIP graph.  Instances are grouped so that a group's state arrays (V and S)
+ * fit a byte budget (by default 128 MiB, half the MI355X's 256 MiB last-level cache);
+ * each group takes all K steps of a block before the next group starts, so the state one
+ * kernel writes is still cached when the other reads it.  Instances join and leave between K-step blocks; each
  * takes exactly its own number of steps.  No counterpart in the reference, which runs
  * one model per process; each instance's arithmetic is that of rsg_step, so its results
  * are bit-identical alone and in any mix.  Instances are ordinary handles (same
@@ -205,9 +208,10 @@ int rsg_last_error(char* buf, size_t len);
  * state and traces with rsg_get_state / rsg_get_traces after rsg_batch_synchronize. */
 typedef struct rsg_batch_s* rsg_batch;
 
-/* steps_per_block: K (0: 50).  grid: workgroups of the persistent launches (0: 8 per CU;
- * -1: one workgroup per work item, the grid sized to the work table's capacity).
- * use_graph: 0 launches every step directly instead of replaying a captured graph. */
+/* steps_per_block: K (0: 50).  grid: 0 or -1 launches one workgroup per work item (the
+ * grid sized to the group's table capacity); n > 0 launches n persistent workgroups that
+ * walk the table.  use_graph: 0 launches every step directly instead of replaying a
+ * captured graph. */
 int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, int32_t use_graph,
                      rsg_batch* out);
 
@@ -220,6 +224,15 @@ int rsg_batch_remove(rsg_batch b, int32_t slot);
 /* Runs up to nblocks K-step blocks (enqueued; see rsg_batch_synchronize).  *unfinished
  * receives the number of instances with steps left. */
 int rsg_batch_run(rsg_batch b, int32_t nblocks, int32_t* unfinished);
+
+/* Scheduling options (results do not depend on them):
+ *   RSG_BATCH_GROUP_BYTES  state bytes per group (>= 0; 0: all instances in one group)
+ *   RSG_BATCH_XCD_RUN      work-item order across the 8 XCDs: -1 table order, 0 one
+ *                          contiguous eighth per XCD, n > 0 runs of n items per XCD in turn
+ *                          (default 64) */
+#define RSG_BATCH_GROUP_BYTES 1
+#define RSG_BATCH_XCD_RUN 2
+int rsg_batch_set_option(rsg_batch b, int32_t option, int64_t value);
 
 int rsg_batch_synchronize(rsg_batch b);
 

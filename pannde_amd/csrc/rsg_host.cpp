@@ -1213,8 +1213,14 @@ struct rsg_batch_s {
   int32_t precision = RSG_FP64;
   int K = 50;
   int grid = 0;         // workgroups of the persistent launches
-  bool dense = false;   // one workgroup per work item (grid = table capacity)
+  bool dense = false;   // one workgroup per work item
   bool use_graph = true;
+  // instances are scheduled in groups whose state arrays (V and S) fit this many bytes;
+  // each group takes all K steps of a block before the next starts (0: one group)
+  int64_t group_bytes = (int64_t)128 << 20;
+  // work-table order across XCDs: -1 identity, 0 one contiguous eighth per XCD, n > 0 runs
+  // of n consecutive items per XCD in turn
+  int32_t xcd_run = 64;
   hipStream_t stream = nullptr;
   struct Slot {
     rsg_instance* h = nullptr;
@@ -1227,14 +1233,25 @@ struct rsg_batch_s {
   int block_v = 256, block_s = 256, cpt_v = 1, cpt_s = 1;
   bool lds_v = true, lds_s = true;
   int32_t max_u_v = 0, max_u_s = 0;
-  // control block on the device: descriptors | velocity work | stress work | amplitudes
-  size_t cap_inst = 0, cap_wv = 0, cap_ws = 0, cap_amp = 0;
+  // control block on the device: descriptors | velocity tables | stress tables | amplitudes.
+  // Each group's table is {n, flags, (instance, block) x n}, 256-byte aligned, with room
+  // for n rounded up to kItemQuantum so small changes of membership keep the layout.
+  size_t cap_inst = 0, cap_wv = 0, cap_ws = 0, cap_amp = 0;  // cap_w*: int32 words
   size_t off_wv = 0, off_ws = 0, off_amp = 0, ctl_bytes = 0;
   char* ctl_dev = nullptr;
   char* ctl_host[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
   bool busy[2] = {false, false};
   int cur = 0;
+  // launch layout of the current block: per group, table word offsets and grids
+  struct Launch {
+    int64_t off_v, off_s;
+    int32_t grid_v, grid_s;
+    bool operator==(const Launch& o) const {
+      return off_v == o.off_v && off_s == o.off_s && grid_v == o.grid_v && grid_s == o.grid_s;
+    }
+  };
+  std::vector<Launch> layout, graph_layout;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool graph_dirty = true;
@@ -1268,7 +1285,17 @@ struct rsg_batch_s {
 
 namespace {
 
+constexpr int32_t kItemQuantum = 256;
+
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+size_t table_words(size_t n) {  // one group's table: header + items, 64-word aligned
+  const size_t q = (n + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
+  return (2 + 2 * q + 63) / 64 * 64;
+}
+
+int64_t state_bytes(const rsg_batch_s* b, const rsg_instance* h) {
+  return (6 * (int64_t)h->nc + 3 * (int64_t)h->nn) * (b->precision == RSG_FP64 ? 8 : 4);
+}
 
 // Grows the control block to hold the current instances (amplitude area for K steps).
 int batch_reserve(rsg_batch_s* b) {
@@ -1280,16 +1307,20 @@ int batch_reserve(rsg_batch_s* b) {
     nws += (size_t)(h->nc + b->block_s * b->cpt_s - 1) / (b->block_s * b->cpt_s);
     namp += std::max<size_t>(1, h->channels.size()) * (size_t)b->K;
   }
-  if (ninst <= b->cap_inst && nwv <= b->cap_wv && nws <= b->cap_ws && namp <= b->cap_amp && b->ctl_dev)
+  // every group table wastes at most 2 + 2 * (kItemQuantum - 1) + 63 words; there are at
+  // most as many groups as instances
+  const size_t slack = (size_t)(2 + 2 * kItemQuantum + 64);
+  const size_t wv = 2 * nwv + slack * std::max<size_t>(1, ninst), ws = 2 * nws + slack * std::max<size_t>(1, ninst);
+  if (ninst <= b->cap_inst && wv <= b->cap_wv && ws <= b->cap_ws && namp <= b->cap_amp && b->ctl_dev)
     return RSG_OK;
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->cap_inst = std::max(b->cap_inst, ninst + ninst / 2 + 4);
-  b->cap_wv = std::max(b->cap_wv, nwv + nwv / 2 + 64);
-  b->cap_ws = std::max(b->cap_ws, nws + nws / 2 + 64);
+  b->cap_wv = std::max(b->cap_wv, wv + wv / 2 + 4 * slack);
+  b->cap_ws = std::max(b->cap_ws, ws + ws / 2 + 4 * slack);
   b->cap_amp = std::max(b->cap_amp, namp + namp / 2 + 64);
   b->off_wv = align256(b->cap_inst * b->desc_size());
-  b->off_ws = b->off_wv + align256((2 + 2 * b->cap_wv) * sizeof(int32_t));
-  b->off_amp = b->off_ws + align256((2 + 2 * b->cap_ws) * sizeof(int32_t));
+  b->off_ws = b->off_wv + align256(b->cap_wv * sizeof(int32_t));
+  b->off_amp = b->off_ws + align256(b->cap_ws * sizeof(int32_t));
   b->ctl_bytes = b->off_amp + align256(b->cap_amp * sizeof(double));
   if (b->ctl_dev) (void)hipFree(b->ctl_dev);
   b->ctl_dev = nullptr;
@@ -1304,31 +1335,32 @@ int batch_reserve(rsg_batch_s* b) {
   return RSG_OK;
 }
 
+// Enqueues one K-step block: every group in turn takes its K steps (velocity then stress
+// per step), replayed from a captured graph while the launch layout stays the same.
 template <typename T>
 int batch_launch_block(rsg_batch_s* b) {
   const auto* desc = (const rsg::BatchDesc<T>*)b->ctl_dev;
   const int32_t* wv = (const int32_t*)(b->ctl_dev + b->off_wv);
   const int32_t* ws = (const int32_t*)(b->ctl_dev + b->off_ws);
-  const int gv = b->dense ? (int)b->cap_wv : b->grid;
-  const int gs = b->dense ? (int)b->cap_ws : b->grid;
   auto enqueue = [&]() -> hipError_t {
-    for (int k = 0; k < b->K; k++) {
-      hipError_t e;
-      if (sizeof(T) == 8) {
-        e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv, k, gv, b->block_v, b->lds_v,
-                                  b->max_u_v, b->cpt_v, true, b->stream);
+    for (const auto& g : b->layout)
+      for (int k = 0; k < b->K; k++) {
+        hipError_t e;
+        if (sizeof(T) == 8) {
+          e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv + g.off_v, k, g.grid_v, b->block_v,
+                                    b->lds_v, b->max_u_v, b->cpt_v, true, b->stream);
+          if (e != hipSuccess) return e;
+          e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws + g.off_s, k, g.grid_s, b->block_s,
+                                    b->lds_s, b->max_u_s, b->cpt_s, false, b->stream);
+        } else {
+          e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv + g.off_v, k, g.grid_v, b->block_v,
+                                    b->lds_v, b->max_u_v, b->cpt_v, true, b->stream);
+          if (e != hipSuccess) return e;
+          e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws + g.off_s, k, g.grid_s, b->block_s,
+                                    b->lds_s, b->max_u_s, b->cpt_s, false, b->stream);
+        }
         if (e != hipSuccess) return e;
-        e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws, k, gs, b->block_s, b->lds_s,
-                                  b->max_u_s, b->cpt_s, false, b->stream);
-      } else {
-        e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv, k, gv, b->block_v, b->lds_v,
-                                  b->max_u_v, b->cpt_v, true, b->stream);
-        if (e != hipSuccess) return e;
-        e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws, k, gs, b->block_s, b->lds_s,
-                                  b->max_u_s, b->cpt_s, false, b->stream);
       }
-      if (e != hipSuccess) return e;
-    }
     return hipSuccess;
   };
   if (!b->use_graph) {
@@ -1336,7 +1368,7 @@ int batch_launch_block(rsg_batch_s* b) {
     if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "batch launch: %s", hipGetErrorString(e));
     return RSG_OK;
   }
-  if (b->graph_dirty || !b->exec) {
+  if (b->graph_dirty || !b->exec || b->graph_layout != b->layout) {
     b->drop_graph();
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = enqueue();
@@ -1348,14 +1380,15 @@ int batch_launch_block(rsg_batch_s* b) {
     }
     b->graph = g;
     HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_layout = b->layout;
     b->graph_dirty = false;
   }
   HIP_TRY(hipGraphLaunch(b->exec, b->stream));
   return RSG_OK;
 }
 
-// Fills one K-step block's control data (descriptors, work tables, amplitudes) for the
-// instances with steps left, uploads it and launches the block.
+// Fills one K-step block's control data (descriptors, group work tables, amplitudes) for
+// the instances with steps left, uploads it and launches the block.
 template <typename T>
 int batch_block(rsg_batch_s* b, int32_t* unfinished) {
   int rc;
@@ -1367,7 +1400,6 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
   int32_t* wv = (int32_t*)(host + b->off_wv);
   int32_t* ws = (int32_t*)(host + b->off_ws);
   double* amp = (double*)(host + b->off_amp);
-  int32_t nwv = 0, nws = 0;
   size_t amp_off = 0;
   // traces that would overflow their device buffer are flushed first (needs the stream idle)
   bool flush = false;
@@ -1413,24 +1445,57 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
       t += h->dt;
     }
     amp_off += nch * (size_t)b->K;
-    const int32_t cv = b->block_v * b->cpt_v, cs = b->block_s * b->cpt_s;
-    const int32_t nbv = (h->nn + cv - 1) / cv, nbs = (h->nc + cs - 1) / cs;
-    for (int32_t q = 0; q < nbv; q++) {
-      wv[2 + 2 * nwv] = (int32_t)i;
-      wv[3 + 2 * nwv] = q;
-      nwv++;
-    }
-    for (int32_t q = 0; q < nbs; q++) {
-      ws[2 + 2 * nws] = (int32_t)i;
-      ws[3 + 2 * nws] = q;
-      nws++;
-    }
   }
-  wv[0] = nwv;
-  wv[1] = 0;
-  ws[0] = nws;
-  ws[1] = 0;
-  if (nwv == 0 && nws == 0) {
+  // groups of consecutive active instances, each within the state budget (an instance
+  // larger than the budget forms a group of its own)
+  const int32_t flags = b->xcd_run < 0 ? 0 : 1 | (b->xcd_run << 8);
+  const int32_t cv = b->block_v * b->cpt_v, cs = b->block_s * b->cpt_s;
+  b->layout.clear();
+  size_t ov = 0, os = 0;
+  std::vector<size_t> members;
+  for (size_t i = 0; i < b->slots.size();) {
+    members.clear();
+    int64_t bytes = 0;
+    int32_t nv = 0, ns = 0;
+    for (; i < b->slots.size(); i++) {
+      if (!kmax[i]) continue;
+      const rsg_instance* h = b->slots[i].h;
+      const int64_t sb = state_bytes(b, h);
+      if (!members.empty() && b->group_bytes > 0 && bytes + sb > b->group_bytes) break;
+      bytes += sb;
+      members.push_back(i);
+      nv += (h->nn + cv - 1) / cv;
+      ns += (h->nc + cs - 1) / cs;
+    }
+    if (members.empty()) break;
+    if (ov + table_words(nv) > b->cap_wv || os + table_words(ns) > b->cap_ws)
+      return fail(RSG_ERR_STATE, "batch work tables overflow");
+    int32_t* tv = wv + ov;
+    int32_t* ts = ws + os;
+    tv[0] = nv;
+    tv[1] = flags;
+    ts[0] = ns;
+    ts[1] = flags;
+    int32_t iv = 0, is = 0;
+    for (size_t e : members) {
+      const rsg_instance* h = b->slots[e].h;
+      const int32_t nbv = (h->nn + cv - 1) / cv, nbs = (h->nc + cs - 1) / cs;
+      for (int32_t q = 0; q < nbv; q++, iv++) {
+        tv[2 + 2 * iv] = (int32_t)e;
+        tv[3 + 2 * iv] = q;
+      }
+      for (int32_t q = 0; q < nbs; q++, is++) {
+        ts[2 + 2 * is] = (int32_t)e;
+        ts[3 + 2 * is] = q;
+      }
+    }
+    const int32_t qv = (nv + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
+    const int32_t qs = (ns + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
+    b->layout.push_back({(int64_t)ov, (int64_t)os, b->dense ? qv : b->grid, b->dense ? qs : b->grid});
+    ov += table_words(nv);
+    os += table_words(ns);
+  }
+  if (b->layout.empty()) {
     *unfinished = 0;
     return RSG_OK;
   }
@@ -1470,16 +1535,10 @@ int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, i
   b->precision = precision;
   b->K = steps_per_block ? steps_per_block : 50;
   b->use_graph = use_graph != 0;
-  if (grid > 0) {
+  if (grid > 0)
     b->grid = grid;
-  } else if (grid < 0) {
+  else
     b->dense = true;
-  } else {
-    int dev = 0, cus = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    b->grid = std::max(1, cus) * 8;
-  }
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   for (int i = 0; i < 2; i++) HIP_TRY(hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming));
   *out = b.release();
@@ -1561,6 +1620,22 @@ int rsg_batch_run(rsg_batch b, int32_t nblocks, int32_t* unfinished) {
   }
   if (unfinished) *unfinished = left;
   return RSG_OK;
+}
+
+int rsg_batch_set_option(rsg_batch b, int32_t option, int64_t value) {
+  if (!b) return fail(RSG_ERR_ARG, "batch is null");
+  switch (option) {
+    case RSG_BATCH_GROUP_BYTES:
+      if (value < 0) return fail(RSG_ERR_ARG, "group bytes %lld < 0", (long long)value);
+      b->group_bytes = value;
+      return RSG_OK;
+    case RSG_BATCH_XCD_RUN:
+      if (value < -1 || value > (1 << 20)) return fail(RSG_ERR_ARG, "xcd run %lld", (long long)value);
+      b->xcd_run = (int32_t)value;
+      return RSG_OK;
+    default:
+      return fail(RSG_ERR_ARG, "unknown batch option %d", option);
+  }
 }
 
 int rsg_batch_synchronize(rsg_batch b) {

@@ -63,6 +63,21 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
   return x * q + (x < r ? x : r) + j;
 }
 
+// Batch work-table position -> item.  flags bit 0 off: identity.  Otherwise, with
+// g = flags >> 8: g == 0 gives XCD x the x-th contiguous eighth of the table (as xcd_block);
+// g > 0 gives each XCD runs of g consecutive items in turn (run r of XCD x is items
+// [(8r + x) g, (8r + x + 1) g)), with the last n mod 8g items in place.
+__device__ __forceinline__ int batch_item(int p, int n, int flags) {
+  if (!(flags & 1)) return p;
+  const int g = flags >> 8;
+  if (g == 0) return xcd_block(p, n);
+  const int n8 = n - n % (8 * g);
+  if (p >= n8) return p;
+  const int x = p & 7, j = p >> 3;
+  const int run = j / g;
+  return (run * 8 + x) * g + (j - run * g);
+}
+
 __device__ __forceinline__ bool is_boxed(double v) { return __builtin_isnan(v); }
 __device__ __forceinline__ bool is_boxed(float v) { return __builtin_isnan(v); }
 __device__ __forceinline__ int unbox(double v) {
@@ -440,7 +455,10 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(S
 }
 
 // Batched launches: a persistent grid walks the work table (work[0] items of (instance,
-// block) after a two-int header) and reads each instance's argument block from desc.
+// block) after a two-int header whose second int holds flags) and reads each instance's
+// argument block from desc.  With flag bit 0 the item positions are remapped like the
+// single-instance block ids, so each XCD takes a contiguous run of the table (neighbouring
+// blocks of one instance share halo entities in that XCD's L2).
 // Step k of the current K-step block runs only for instances with k < kmax, so instances
 // of different lengths stop exactly at their own step count.  The table and the
 // descriptors live in device memory, so a captured graph of these launches stays valid
@@ -450,9 +468,10 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_batc
     const BatchDesc<T>* __restrict__ desc, const int32_t* __restrict__ work, int k) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int n_work = work[0];
+  const int flags = work[1];
   const int2* items = reinterpret_cast<const int2*>(work + 2);
-  for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
-    const int2 it = items[w];
+  for (int p = blockIdx.x; p < n_work; p += gridDim.x) {
+    const int2 it = items[batch_item(p, n_work, flags)];
     const BatchDesc<T>* d = desc + it.x;
     if (k < d->kmax) {
       VelocityArgs<T> a = d->v;
@@ -469,9 +488,10 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_batch_ke
     const BatchDesc<T>* __restrict__ desc, const int32_t* __restrict__ work, int k) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int n_work = work[0];
+  const int flags = work[1];
   const int2* items = reinterpret_cast<const int2*>(work + 2);
-  for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
-    const int2 it = items[w];
+  for (int p = blockIdx.x; p < n_work; p += gridDim.x) {
+    const int2 it = items[batch_item(p, n_work, flags)];
     const BatchDesc<T>* d = desc + it.x;
     if (k < d->kmax) {
       StressArgs<T> a = d->s;

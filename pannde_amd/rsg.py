@@ -25,6 +25,7 @@ LIB_PATH = os.environ.get("PANNDE_RSG_LIB") or os.path.join(HERE, "libpannde_rsg
 RSG_OK = 0
 RSG_FP64, RSG_FP32 = 0, 1
 RSG_FORCE, RSG_MOMENT = 0, 1
+RSG_BATCH_GROUP_BYTES, RSG_BATCH_XCD_RUN = 1, 2
 STATE_V = ("Vx", "Vy", "Vz")
 STATE_S = ("Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy")
 
@@ -34,7 +35,7 @@ EXPORTS = ("rsg_create", "rsg_set_channels", "rsg_set_sensors", "rsg_step", "rsg
            "rsg_get_kernel_times", "rsg_device_bytes", "rsg_destroy", "rsg_last_error",
            "rsg_disc_support", "rsg_signal_eval", "rsg_create_ex", "rsg_get_tuning", "rsg_batch_create",
            "rsg_batch_add", "rsg_batch_remove", "rsg_batch_run", "rsg_batch_synchronize", "rsg_batch_slot_status",
-           "rsg_batch_destroy")
+           "rsg_batch_destroy", "rsg_batch_set_option")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -128,6 +129,7 @@ def lib():
         L.rsg_batch_slot_status.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
                                             ctypes.POINTER(ctypes.c_int64)]
         L.rsg_batch_destroy.argtypes = [vp]
+        L.rsg_batch_set_option.argtypes = [vp, ctypes.c_int32, ctypes.c_int64]
         for name in EXPORTS:
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
@@ -350,15 +352,24 @@ class DeviceSolver:
 
 
 class Batch:
-    """Many DeviceSolver instances advanced together (rsg_batch_*): one persistent launch
-    per kernel per step over all of them, K-step blocks replayed from a HIP graph."""
+    """Many DeviceSolver instances advanced together (rsg_batch_*): one launch per kernel
+    per step over each group of instances, K-step blocks replayed from a HIP graph.
+    group_bytes / xcd_run set the scheduling options (None keeps the library default)."""
 
-    def __init__(self, precision=RSG_FP64, steps_per_block=50, grid=0, use_graph=True):
+    def __init__(self, precision=RSG_FP64, steps_per_block=50, grid=0, use_graph=True, group_bytes=None,
+                 xcd_run=None):
         h = ctypes.c_void_p()
         _check(lib().rsg_batch_create(int(precision), int(steps_per_block), int(grid), 1 if use_graph else 0,
                                       ctypes.byref(h)))
         self._h = h
         self.members = {}
+        if group_bytes is not None:
+            self.set_option(RSG_BATCH_GROUP_BYTES, group_bytes)
+        if xcd_run is not None:
+            self.set_option(RSG_BATCH_XCD_RUN, xcd_run)
+
+    def set_option(self, option: int, value: int):
+        _check(lib().rsg_batch_set_option(self._h, int(option), int(value)))
 
     def add(self, solver: DeviceSolver, nsteps: int) -> int:
         slot = ctypes.c_int32(0)
