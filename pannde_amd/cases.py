@@ -70,6 +70,7 @@ class Case:
     moment_form: int = 0
     sensors: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
     seed: int = 0
+    meta: dict = dataclasses.field(default_factory=dict)  # generator parameters (crack, ...)
 
     @property
     def n_nodes(self) -> int:
@@ -246,10 +247,11 @@ def cfrp_ply_c():
 
 
 def cfrp_plate(seed=2, nx=256, ny=256, cells_per_ply=4, dx=5.0e-5, nsteps=2000, n_holes=5,
-               n_sensors=16, renumber=True, xd_radius=2.0e-3):
+               n_sensors=16, renumber=True, xd_radius=2.0e-3, crack=None):
     """Orthotropic CFRP laminate, [0/45/90/-45]2s, 16 plies of ``cells_per_ply`` cells,
     rho 1580, seeded through-holes of radius 1-3 mm (cells removed, orphan nodes
-    dropped), node ids seeded-permuted then RCM-renumbered, 2 MHz 5-cycle burst."""
+    dropped), node ids seeded-permuted then RCM-renumbered, 2 MHz 5-cycle burst.
+    ``crack`` (a Crack) is opened on the top face before renumbering."""
     rng = np.random.default_rng(seed)
     nplies = len(CFRP_LAYUP)
     nz = nplies * cells_per_ply
@@ -274,6 +276,9 @@ def cfrp_plate(seed=2, nx=256, ny=256, cells_per_ply=4, dx=5.0e-5, nsteps=2000, 
         keep2d &= (xc[:, None] - hx) ** 2 + (yc[None, :] - hy) ** 2 >= r * r
     keep = np.repeat(keep2d[:, :, None], nz, axis=2)
     x, y, z, cells, ijk = structured_mesh(nx, ny, nz, dx, dx, dx, keep=keep)
+    n_dup = 0
+    if crack is not None:
+        x, y, z, cells, n_dup = surface_crack(x, y, z, cells, ijk, (dx, dx, dx), crack)
     ply_c = np.stack([rotate_c_about_z(cfrp_ply_c(), a) for a in CFRP_LAYUP])
     c_cells = ply_c[ijk[:, 2] // cells_per_ply]
     rho = 1580.0
@@ -287,7 +292,150 @@ def cfrp_plate(seed=2, nx=256, ny=256, cells_per_ply=4, dx=5.0e-5, nsteps=2000, 
     return Case(name="cfrp_plate_%dx%dx%d" % (nx, ny, nz), x=x, y=y, z=z, cells=cells,
                 rho=Field(FIELD_CELL, np.full(cells.shape[0], rho)), cij=cij_cell_fields(c_cells),
                 dt=dt, nsteps=nsteps, force=[disc],
-                sensors=seeded_sensors(rng, top_face_nodes(z), n_sensors), seed=seed)
+                sensors=seeded_sensors(rng, top_face_nodes(z), n_sensors), seed=seed,
+                meta=dict(crack=crack, n_dup=n_dup, holes=holes))
+
+
+@dataclasses.dataclass
+class Crack:
+    """A planar, surface-breaking crack: vertical plane through ``center`` (on the top
+    face), normal at ``angle_deg`` from +x in the x-y plane, ``length`` along the surface
+    and ``depth`` below it."""
+
+    center: Sequence[float]
+    angle_deg: float
+    length: float
+    depth: float
+
+
+def surface_crack(x, y, z, cells, ijk, d, crack: Crack, origin=(0.0, 0.0, 0.0)):
+    """Opens ``crack`` in a right-hex mesh on a grid of spacing d = (dx, dy, dz) by
+    duplicating nodes, as a mesh file describes a crack (coincident nodes on the two
+    faces, each face's cells using their own copy).
+
+    Every cell gets the side of the crack plane its centre lies on.  A face between two
+    cells of opposite sides is cracked when its centre is within the crack rectangle.  A
+    node is duplicated when every opposite-side face around it is cracked (so the crack
+    tip stays bonded and the mouth at the surface opens); the negative side's cells take
+    the copy, appended after the existing nodes.  Returns (x, y, z, cells, n_dup)."""
+    dx, dy, dz = d
+    th = math.radians(crack.angle_deg)
+    nrm = np.array([math.cos(th), math.sin(th), 0.0])
+    tan = np.array([-math.sin(th), math.cos(th), 0.0])
+    cx, cy, ztop = (float(v) for v in crack.center)
+    half = 0.5 * crack.length
+    # cell region: the crack rectangle's bounding box plus two cells
+    ext = np.abs(tan[:2]) * half
+    lo = np.array([cx - ext[0] - 2 * dx, cy - ext[1] - 2 * dy, ztop - crack.depth - 2 * dz])
+    hi = np.array([cx + ext[0] + 2 * dx, cy + ext[1] + 2 * dy, ztop + 2 * dz])
+    g_lo = np.floor((lo - np.asarray(origin)) / np.array(d)).astype(np.int64)
+    g_hi = np.ceil((hi - np.asarray(origin)) / np.array(d)).astype(np.int64)
+    g_lo = np.maximum(g_lo, ijk.min(axis=0))
+    g_hi = np.minimum(g_hi, ijk.max(axis=0) + 1)
+    if np.any(g_hi <= g_lo):
+        return x, y, z, cells, 0
+    shape = tuple(int(v) for v in g_hi - g_lo)
+    inreg = np.all((ijk >= g_lo) & (ijk < g_hi), axis=1)
+    reg_cells = np.nonzero(inreg)[0]
+    loc = ijk[reg_cells] - g_lo
+    cgrid = np.full(shape, -1, dtype=np.int64)
+    cgrid[loc[:, 0], loc[:, 1], loc[:, 2]] = reg_cells
+    # side of each region cell (+1 / -1) from its centre
+    ii, jj, kk = np.meshgrid(*(np.arange(n) for n in shape), indexing="ij")
+    gx = origin[0] + (g_lo[0] + ii + 0.5) * dx
+    gy = origin[1] + (g_lo[1] + jj + 0.5) * dy
+    side = np.where((gx - cx) * nrm[0] + (gy - cy) * nrm[1] >= 0.0, 1, -1)
+    exists = cgrid >= 0
+    ncrack = np.zeros(tuple(n + 1 for n in shape), dtype=np.int32)
+    nbond = np.zeros_like(ncrack)
+    for a in range(3):
+        sl0 = [slice(None)] * 3
+        sl1 = [slice(None)] * 3
+        sl0[a] = slice(0, shape[a] - 1)
+        sl1[a] = slice(1, shape[a])
+        sl0, sl1 = tuple(sl0), tuple(sl1)
+        opp = exists[sl0] & exists[sl1] & (side[sl0] != side[sl1])
+        fi, fj, fk = np.nonzero(opp)
+        # face centre: the shared boundary between the two cells
+        f = np.stack([fi, fj, fk], axis=1).astype(np.float64) + 0.5
+        f[:, a] += 0.5
+        fxyz = np.asarray(origin) + (f + g_lo) * np.array(d)
+        along = (fxyz[:, 0] - cx) * tan[0] + (fxyz[:, 1] - cy) * tan[1]
+        below = ztop - fxyz[:, 2]
+        inside = (np.abs(along) <= half) & (below <= crack.depth)
+        # the face's 4 nodes in region-node coordinates
+        b = np.stack([fi, fj, fk], axis=1)
+        b[:, a] += 1
+        others = [q for q in range(3) if q != a]
+        for d0 in (0, 1):
+            for d1 in (0, 1):
+                nidx = b.copy()
+                nidx[:, others[0]] += d0
+                nidx[:, others[1]] += d1
+                np.add.at(ncrack, (nidx[inside, 0], nidx[inside, 1], nidx[inside, 2]), 1)
+                np.add.at(nbond, (nidx[~inside, 0], nidx[~inside, 1], nidx[~inside, 2]), 1)
+    dup_grid = (ncrack > 0) & (nbond == 0)
+    if not dup_grid.any():
+        return x, y, z, cells, 0
+    # node ids of the duplicated grid nodes (through the region's cells' vertices)
+    offs = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    node_at = np.full(ncrack.shape, -1, dtype=np.int64)
+    for v, (a0, a1, a2) in enumerate(offs):
+        node_at[loc[:, 0] + a0, loc[:, 1] + a1, loc[:, 2] + a2] = cells[reg_cells, v]
+    old = node_at[dup_grid]
+    assert np.all(old >= 0)
+    nn = x.shape[0]
+    new_of = np.full(nn, -1, dtype=np.int64)
+    new_of[old] = nn + np.arange(old.shape[0])
+    cells = cells.copy()
+    neg = reg_cells[side[loc[:, 0], loc[:, 1], loc[:, 2]] < 0]
+    sub = cells[neg]
+    rep = new_of[sub]
+    cells[neg] = np.where(rep >= 0, rep, sub).astype(np.int32)
+    return (np.concatenate([x, x[old]]), np.concatenate([y, y[old]]), np.concatenate([z, z[old]]), cells,
+            int(old.shape[0]))
+
+
+def ti_bracket(seed=5, leg=512, width=96, height=96, dx=5.0e-5, nsteps=4000, crack_length=None,
+               crack_depth=None, crack_angle=None, n_sensors=16, xd_radius=1.5e-3):
+    """L-shaped Ti-6Al-4V bracket (rho 4430, lambda 90 GPa, mu 44 GPa): leg A is
+    leg x width x height cells along x, leg B width x leg x height cells along y from
+    y = width, both ``height`` cells tall.  A seeded planar surface crack on leg A's top
+    face (length U(0.5, 5) mm, depth U(0.2, 2) mm, orientation U(0, 180) deg unless
+    given), modelled by duplicated nodes; a 10 MHz 3-cycle burst on a disc on the top face
+    of leg A; 16 seeded top-face sensors."""
+    rng = np.random.default_rng(seed)
+    nx, ny = leg, width + leg
+    ci = np.arange(nx)[:, None]
+    cj = np.arange(ny)[None, :]
+    keep2d = (cj < width) | (ci < width)
+    keep = np.repeat(keep2d[:, :, None], height, axis=2)
+    x, y, z, cells, ijk = structured_mesh(nx, ny, height, dx, dx, dx, keep=keep)
+    L = leg * dx
+    W = width * dx
+    H = height * dx
+    length = crack_length if crack_length is not None else rng.uniform(0.5e-3, 5.0e-3)
+    depth = crack_depth if crack_depth is not None else rng.uniform(0.2e-3, 2.0e-3)
+    depth = min(depth, 0.8 * H)
+    angle = crack_angle if crack_angle is not None else rng.uniform(0.0, 180.0)
+    ccx = rng.uniform(0.55 * L, 0.8 * L)
+    ccy = rng.uniform(0.3 * W, 0.7 * W)
+    crack = Crack((ccx, ccy, H), angle, length, depth)
+    x, y, z, cells, n_dup = surface_crack(x, y, z, cells, ijk, (dx, dx, dx), crack)
+    mat = TI64
+    c6 = isotropic_c(mat["lam"], mat["mu"])
+    nc = cells.shape[0]
+    dt = cfl_dt(math.sqrt((mat["lam"] + 2 * mat["mu"]) / mat["rho"]), dx, dx, dx)
+    t, v = tone_burst(10.0e6, 3, dt)
+    r = min(xd_radius, 0.4 * W)
+    disc = Disc(2, (rng.uniform(0.25 * L, 0.4 * L), 0.5 * W, H), r, dx, t, v)
+    top = top_face_nodes(z)
+    case = Case(name="ti_bracket_%dx%dx%d" % (leg, width, height), x=x, y=y, z=z, cells=cells,
+                rho=Field(FIELD_CELL, np.full(nc, mat["rho"])),
+                cij=[Field(FIELD_CONSTANT, np.array([c6[i, j]])) for i, j in CIJ_PAIRS],
+                dt=dt, nsteps=nsteps, force=[disc], sensors=seeded_sensors(rng, top, n_sensors), seed=seed)
+    case.meta.update(crack=crack, n_dup=n_dup)
+    return case
 
 
 def water_column_block(seed=3, nx=12, ny=10, nz=6, dx=1.0e-4, nsteps=40):

@@ -7,7 +7,7 @@ oracle/_ref/ref_rsg, i.e. the reference headers compiled as they lie in the refe
 tree (``make -C oracle ref``), never from this project's code.
 
 Run from the repository root in a tree where the reference is present:
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture names]   (default: all)
 """
 from __future__ import annotations
 
@@ -47,6 +47,11 @@ def fixture_cases():
                               d.signal_t, 0.3 * d.signal_v))
     c.name = "three_transducers"
     out.append((c, 1))
+    # L-bracket with a surface crack at 60 degrees (duplicated crack-face nodes)
+    c = cases.ti_bracket(seed=16, leg=24, width=8, height=6, dx=2.5e-4, nsteps=30, crack_length=1.2e-3,
+                         crack_depth=0.6e-3, crack_angle=60.0, n_sensors=8)
+    c.name = "ti_bracket_crack"
+    out.append((c, 2))
     return out
 
 
@@ -76,5 +81,7 @@ def save(case, ranks):
 if __name__ == "__main__":
     if not orc.have_reference():
         sys.exit("oracle/_ref/ref_rsg is missing: run make -C oracle ref where the reference is present")
+    names = set(sys.argv[1:])
     for c, r in fixture_cases():
-        save(c, r)
+        if not names or c.name in names:
+            save(c, r)

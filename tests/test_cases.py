@@ -1,0 +1,111 @@
+"""Workload generators: the surface crack (duplicated crack-face nodes) and the L-bracket.
+
+The crack is pinned three ways: an axis-aligned crack duplicates exactly the nodes a hand
+count gives; the copies coincide with their originals and no cell uses both; and a crack
+through the whole cross-section splits the body in two, so the oracle must leave every
+velocity and stress of the piece without the source exactly zero while the same job
+without the crack reaches it (a property of the scheme, not of this code)."""
+import math
+
+import numpy as np
+
+from oracle import oracle_rsg as orc
+from pannde_amd import cases
+
+
+def block(nx, ny, nz, d=1.0):
+    return cases.structured_mesh(nx, ny, nz, d, d, d)
+
+
+def test_axis_aligned_crack_duplicates_exact_nodes():
+    x, y, z, cells, ijk = block(20, 12, 8)
+    # plane y = 6 (normal +y), 8 cells along x from x = 6 to 14, 4 cells deep from z = 8
+    crack = cases.Crack((10.0, 6.0, 8.0), 90.0, 8.0, 4.0)
+    x2, y2, z2, c2, n_dup = cases.surface_crack(x, y, z, cells, ijk, (1.0, 1.0, 1.0), crack)
+    # node columns x = 7..13 (the ends stay bonded), rows z = 5..8 (the bottom row z = 4 stays
+    # bonded, the mouth at z = 8 opens)
+    assert n_dup == 7 * 4
+    new = np.arange(x.shape[0], x2.shape[0])
+    assert set(np.round(x2[new]).astype(int)) == set(range(7, 14))
+    assert set(np.round(z2[new]).astype(int)) == set(range(5, 9))
+    assert np.all(y2[new] == 6.0)
+    # cells on the negative side (y < 6) took the copies, the others kept the originals
+    uses_copy = np.isin(c2, new).any(axis=1)
+    assert np.all(ijk[uses_copy, 1] == 5)
+    assert not np.isin(c2[ijk[:, 1] == 6], new).any()
+
+
+def faces_shared(cells):
+    f = [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)]
+    keys = np.concatenate([np.sort(cells[:, q], axis=1) for q in f])
+    _, cnt = np.unique(keys, axis=0, return_counts=True)
+    return int((cnt == 2).sum())
+
+
+def test_oblique_crack_copies_coincide_and_split_cells():
+    c0 = cases.ti_bracket(seed=5, leg=48, width=12, height=12, crack_length=1e-9, crack_depth=1e-9,
+                          crack_angle=35.0)
+    c = cases.ti_bracket(seed=5, leg=48, width=12, height=12, crack_length=1.0e-3, crack_depth=0.3e-3,
+                         crack_angle=35.0)
+    assert c0.meta["n_dup"] == 0 and c.meta["n_dup"] > 0
+    nn0 = c0.n_nodes
+    assert c.n_nodes == nn0 + c.meta["n_dup"] and c.n_cells == c0.n_cells
+    # every copy sits on an original; a cell uses either the original or the copy
+    orig = {}
+    for k in range(nn0, c.n_nodes):
+        m = np.nonzero((c.x[:nn0] == c.x[k]) & (c.y[:nn0] == c.y[k]) & (c.z[:nn0] == c.z[k]))[0]
+        assert m.shape[0] == 1
+        orig[k] = m[0]
+    for row in c.cells:
+        s = set(row.tolist())
+        assert not any(k in s and orig[k] in s for k in orig)
+    # only cells next to the crack changed, and the mesh lost shared faces
+    changed = np.any(c.cells != c0.cells, axis=1)
+    assert 0 < changed.sum() < 4 * c.meta["n_dup"]
+    assert faces_shared(c.cells) < faces_shared(c0.cells)
+    used = np.zeros(c.n_nodes, dtype=bool)
+    used[c.cells.ravel()] = True
+    assert used.all()
+
+
+def through_cut_job(cut):
+    dx = 1.0e-4
+    nx, ny, nz = 24, 6, 5
+    x, y, z, cells, ijk = block(nx, ny, nz, dx)
+    if cut:  # plane x = 14 dx through the whole width and height
+        x, y, z, cells, n = cases.surface_crack(x, y, z, cells, ijk, (dx, dx, dx),
+                                                cases.Crack((14 * dx, 3 * dx, nz * dx), 0.0, 4 * ny * dx,
+                                                            4 * nz * dx))
+        assert n == (ny + 1) * (nz + 1)
+    rho, lam, mu = 2700.0, 51.0e9, 26.0e9
+    dt = cases.cfl_dt(math.sqrt((lam + 2 * mu) / rho), dx, dx, dx)
+    t, v = cases.tone_burst(5.0e6, 3, dt)
+    c6 = cases.isotropic_c(lam, mu)
+    nc = cells.shape[0]
+    return cases.Case(name="cut", x=x, y=y, z=z, cells=cells, rho=cases.Field(cases.FIELD_CELL, np.full(nc, rho)),
+                      cij=cases.cij_cell_fields(np.broadcast_to(c6, (nc, 6, 6))), dt=dt, nsteps=90,
+                      force=[cases.Disc(2, (6 * dx, 3 * dx, nz * dx), 2.5 * dx, dx, t, v)]), ijk
+
+
+def test_through_crack_isolates_the_far_piece():
+    for cut in (False, True):
+        case, ijk = through_cut_job(cut)
+        V, S, _, _ = orc.run_oracle(case)
+        far_cells = ijk[:, 0] >= 14
+        far_nodes = np.unique(case.cells[far_cells])
+        near_nodes = np.unique(case.cells[~far_cells])
+        assert np.abs(V[:, near_nodes]).max() > 0
+        if cut:
+            assert not np.intersect1d(far_nodes, near_nodes).size
+            assert np.all(V[:, far_nodes] == 0.0) and np.all(S[:, far_cells] == 0.0)
+        else:
+            assert np.abs(V[:, far_nodes]).max() > 0 and np.abs(S[:, far_cells]).max() > 0
+
+
+def test_bracket_shape_and_seeded_crack():
+    c = cases.ti_bracket(seed=7, leg=40, width=10, height=8)
+    assert c.n_cells == 2 * 40 * 10 * 8
+    cr = c.meta["crack"]
+    assert 0.5e-3 <= cr.length <= 5.0e-3 and 0.0 <= cr.angle_deg < 180.0
+    again = cases.ti_bracket(seed=7, leg=40, width=10, height=8)
+    assert np.array_equal(c.cells, again.cells) and c.meta["n_dup"] == again.meta["n_dup"]
