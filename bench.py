@@ -13,7 +13,12 @@ timed steps; roofline = the stress kernel's algorithmic bytes per launch over it
 launch duration (HIP events on the solver's stream, same timed region); cpu_baseline =
 the reference solver itself (oracle/_ref/ref_rsg, built from the reference headers) under
 mpirun on this host's cores, on a bounded sample of the same job; fp32 = the mixed-
-precision form on the same job with its sensor-trace error against FP64.
+precision form on the same job with its sensor-trace error against FP64; batched = 64
+independent instances (32^3..128^3 cells, 1500 steps, 50-step graph blocks) advanced
+together in FP32, every instance's traces checked against the same batch run in FP64;
+served = a bounded sample of the seeded inspection stream (cracked brackets and plates,
+0.5M-8M cells, 1000-4000 steps, Poisson arrivals) through the continuous-batching
+server, with solves/hr, p50/p99 latency and the FP64 spot checks.
 """
 import argparse
 import json
@@ -67,6 +72,69 @@ def run_timed(rsg, case, precision, warmup, steps):
                 dev_bytes=dev_bytes)
 
 
+def create_all(rsg, case_list, precision, threads=8):
+    import concurrent.futures as cf
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(lambda c: rsg.DeviceSolver.from_case(c, precision), case_list))
+
+
+def batched(rsg, cases, seed, steps_per_block=50):
+    """All instances in one batch for their full step count, FP32 timed; the same batch in
+    FP64 for the per-trace error."""
+    from pannde_amd import serve
+
+    inst = cases.batch_instances(seed=seed)
+    nsteps = inst[0].nsteps
+    cells = sum(c.n_cells for c in inst)
+    out = {"instances": len(inst), "cells": cells, "steps": nsteps, "steps_per_block": steps_per_block}
+    traces = {}
+    for prec, tag in ((rsg.RSG_FP32, "fp32"), (rsg.RSG_FP64, "fp64")):
+        t0 = time.perf_counter()
+        solvers = create_all(rsg, inst, prec)
+        t_create = time.perf_counter() - t0
+        b = rsg.Batch(prec, steps_per_block=steps_per_block)
+        for s in solvers:
+            b.add(s, nsteps)
+        b.synchronize()
+        t0 = time.perf_counter()
+        left = b.run(10 ** 6)
+        b.synchronize()
+        dt = time.perf_counter() - t0
+        assert left == 0
+        traces[tag] = [s.traces() for s in solvers]
+        for slot in list(b.members):
+            b.remove(slot)
+        b.close()
+        for s in solvers:
+            s.close()
+        out[tag] = {"value": round(cells * nsteps / dt / 1e9, 4), "seconds": round(dt, 3),
+                    "create_seconds": round(t_create, 2)}
+        log("bench: batched %s: %d instances, %d cells, %d steps in %.3f s -> %.3f Gcell-steps/s (create %.1f s)"
+            % (tag, len(inst), cells, nsteps, dt, out[tag]["value"], t_create))
+    errs = [float(serve.rel_l2_traces(a, b).max()) for a, b in zip(traces["fp32"], traces["fp64"])]
+    out["max_trace_rel_l2_fp32_vs_fp64"] = float("%.3e" % max(errs))
+    return out
+
+
+def served(rsg, n, rate, seed):
+    from pannde_amd import serve
+
+    reqs = serve.request_stream(seed=seed, n=n, rate=rate)
+    srv = serve.Server(log=log)
+    rep = srv.serve(reqs)
+    keys = ("requests", "elapsed_s", "solves_per_hour", "latency_p50_s", "latency_p99_s", "latency_mean_s",
+            "gcell_steps_per_s", "gcell_steps_per_s_with_checks", "gpu_busy_fraction", "prep_seconds_mean",
+            "spot_checks", "spot_check_max_trace_rel_l2")
+    out = {k: (float("%.4g" % rep[k]) if isinstance(rep[k], float) else rep[k]) for k in keys}
+    out.update(arrival_rate_per_s=rate, seed=seed, hbm_budget_gib=srv.hbm_budget / (1 << 30),
+               max_resident_cells=srv.max_resident_cells, steps_per_block=srv.K)
+    log("bench: served %d requests in %.1f s: %.0f solves/hr, p50 %.2f s, p99 %.2f s, %.2f Gcell-steps/s"
+        % (n, rep["elapsed_s"], rep["solves_per_hour"], rep["latency_p50_s"], rep["latency_p99_s"],
+           rep["gcell_steps_per_s"]))
+    return out
+
+
 def cpu_baseline(case, ranks, steps):
     from oracle import oracle_rsg as orc
 
@@ -96,6 +164,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--batch-seed", type=int, default=4)
+    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-rate", type=float, default=2.5)
+    ap.add_argument("--serve-seed", type=int, default=9)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: the DeviceSolver holds the whole mesh on one GPU; multi-GPU runs are not part of "
@@ -165,6 +239,16 @@ def main():
                        "stress_kernel_gbs": round(STRESS_BYTES["f32"] * nc / s32 / 1e9, 1)}
         log("bench: FP32 %.3f Gcell-steps/s (x%.2f), trace err %.2e, wavefield err %.2e"
             % (v32, v32 / value, max(tr_err), wf_err))
+
+    if not args.no_batched:
+        log("bench: batched FP32 instances (seed %d)" % args.batch_seed)
+        out["batched"] = batched(rsg, cases, args.batch_seed)
+        out["batched"]["ratio_fp32_batched_vs_fp64_plain"] = round(out["batched"]["fp32"]["value"] / value, 3)
+
+    if not args.no_served:
+        log("bench: served stream, %d requests at %.2f per second" % (args.serve_requests, args.serve_rate))
+        out["served"] = served(rsg, args.serve_requests, args.serve_rate, args.serve_seed)
+        out["served"]["ratio_served_vs_fp64_plain"] = round(out["served"]["gcell_steps_per_s"] / value, 3)
 
     if not args.no_cpu_baseline:
         ranks = args.cpu_ranks or max(1, min(16, os.cpu_count() or 1))

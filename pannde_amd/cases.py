@@ -247,11 +247,12 @@ def cfrp_ply_c():
 
 
 def cfrp_plate(seed=2, nx=256, ny=256, cells_per_ply=4, dx=5.0e-5, nsteps=2000, n_holes=5,
-               n_sensors=16, renumber=True, xd_radius=2.0e-3, crack=None):
+               n_sensors=16, renumber=True, xd_radius=2.0e-3, crack=None, xd_center=None):
     """Orthotropic CFRP laminate, [0/45/90/-45]2s, 16 plies of ``cells_per_ply`` cells,
     rho 1580, seeded through-holes of radius 1-3 mm (cells removed, orphan nodes
     dropped), node ids seeded-permuted then RCM-renumbered, 2 MHz 5-cycle burst.
-    ``crack`` (a Crack) is opened on the top face before renumbering."""
+    ``crack`` (a Crack) is opened on the top face before renumbering; ``xd_center`` (x, y)
+    moves the transducer from the plate centre (holes keep clear of it)."""
     rng = np.random.default_rng(seed)
     nplies = len(CFRP_LAYUP)
     nz = nplies * cells_per_ply
@@ -259,7 +260,7 @@ def cfrp_plate(seed=2, nx=256, ny=256, cells_per_ply=4, dx=5.0e-5, nsteps=2000, 
     xc = (np.arange(nx) + 0.5) * dx
     yc = (np.arange(ny) + 0.5) * dx
     keep2d = np.ones((nx, ny), dtype=bool)
-    center = (lx / 2, ly / 2, nz * dx)
+    center = (lx / 2, ly / 2, nz * dx) if xd_center is None else (xd_center[0], xd_center[1], nz * dx)
     holes = []
     scale = min(lx, ly) / 12.8e-3
     for _ in range(n_holes):
@@ -397,13 +398,13 @@ def surface_crack(x, y, z, cells, ijk, d, crack: Crack, origin=(0.0, 0.0, 0.0)):
 
 
 def ti_bracket(seed=5, leg=512, width=96, height=96, dx=5.0e-5, nsteps=4000, crack_length=None,
-               crack_depth=None, crack_angle=None, n_sensors=16, xd_radius=1.5e-3):
+               crack_depth=None, crack_angle=None, n_sensors=16, xd_radius=1.5e-3, xd_center=None):
     """L-shaped Ti-6Al-4V bracket (rho 4430, lambda 90 GPa, mu 44 GPa): leg A is
     leg x width x height cells along x, leg B width x leg x height cells along y from
     y = width, both ``height`` cells tall.  A seeded planar surface crack on leg A's top
     face (length U(0.5, 5) mm, depth U(0.2, 2) mm, orientation U(0, 180) deg unless
     given), modelled by duplicated nodes; a 10 MHz 3-cycle burst on a disc on the top face
-    of leg A; 16 seeded top-face sensors."""
+    of leg A (at ``xd_center`` (x, y) if given); 16 seeded top-face sensors."""
     rng = np.random.default_rng(seed)
     nx, ny = leg, width + leg
     ci = np.arange(nx)[:, None]
@@ -428,7 +429,8 @@ def ti_bracket(seed=5, leg=512, width=96, height=96, dx=5.0e-5, nsteps=4000, cra
     dt = cfl_dt(math.sqrt((mat["lam"] + 2 * mat["mu"]) / mat["rho"]), dx, dx, dx)
     t, v = tone_burst(10.0e6, 3, dt)
     r = min(xd_radius, 0.4 * W)
-    disc = Disc(2, (rng.uniform(0.25 * L, 0.4 * L), 0.5 * W, H), r, dx, t, v)
+    xd = (rng.uniform(0.25 * L, 0.4 * L), 0.5 * W) if xd_center is None else xd_center
+    disc = Disc(2, (xd[0], xd[1], H), r, dx, t, v)
     top = top_face_nodes(z)
     case = Case(name="ti_bracket_%dx%dx%d" % (leg, width, height), x=x, y=y, z=z, cells=cells,
                 rho=Field(FIELD_CELL, np.full(nc, mat["rho"])),

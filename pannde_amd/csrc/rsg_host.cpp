@@ -1336,49 +1336,49 @@ int batch_reserve(rsg_batch_s* b) {
 }
 
 // Enqueues one K-step block: every group in turn takes its K steps (velocity then stress
-// per step), replayed from a captured graph while the launch layout stays the same.
+// per step).  The launches form a chain of kernel nodes in a graph built explicitly (not
+// by stream capture, so other threads' HIP calls cannot disturb it) and replayed while
+// the launch layout stays the same.
 template <typename T>
 int batch_launch_block(rsg_batch_s* b) {
   const auto* desc = (const rsg::BatchDesc<T>*)b->ctl_dev;
   const int32_t* wv = (const int32_t*)(b->ctl_dev + b->off_wv);
   const int32_t* ws = (const int32_t*)(b->ctl_dev + b->off_ws);
-  auto enqueue = [&]() -> hipError_t {
+  auto emit = [&](hipGraph_t graph, hipGraphNode_t* node) -> hipError_t {
     for (const auto& g : b->layout)
       for (int k = 0; k < b->K; k++) {
         hipError_t e;
         if (sizeof(T) == 8) {
           e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv + g.off_v, k, g.grid_v, b->block_v,
-                                    b->lds_v, b->max_u_v, b->cpt_v, true, b->stream);
+                                    b->lds_v, b->max_u_v, b->cpt_v, true, b->stream, graph, node);
           if (e != hipSuccess) return e;
           e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws + g.off_s, k, g.grid_s, b->block_s,
-                                    b->lds_s, b->max_u_s, b->cpt_s, false, b->stream);
+                                    b->lds_s, b->max_u_s, b->cpt_s, false, b->stream, graph, node);
         } else {
           e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv + g.off_v, k, g.grid_v, b->block_v,
-                                    b->lds_v, b->max_u_v, b->cpt_v, true, b->stream);
+                                    b->lds_v, b->max_u_v, b->cpt_v, true, b->stream, graph, node);
           if (e != hipSuccess) return e;
           e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws + g.off_s, k, g.grid_s, b->block_s,
-                                    b->lds_s, b->max_u_s, b->cpt_s, false, b->stream);
+                                    b->lds_s, b->max_u_s, b->cpt_s, false, b->stream, graph, node);
         }
         if (e != hipSuccess) return e;
       }
     return hipSuccess;
   };
   if (!b->use_graph) {
-    hipError_t e = enqueue();
+    hipError_t e = emit(nullptr, nullptr);
     if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "batch launch: %s", hipGetErrorString(e));
     return RSG_OK;
   }
   if (b->graph_dirty || !b->exec || b->graph_layout != b->layout) {
     b->drop_graph();
-    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue();
-    hipGraph_t g = nullptr;
-    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-    if (e != hipSuccess || e2 != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return fail(RSG_ERR_DEVICE, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    HIP_TRY(hipGraphCreate(&b->graph, 0));
+    hipGraphNode_t node = nullptr;
+    hipError_t e = emit(b->graph, &node);
+    if (e != hipSuccess) {
+      b->drop_graph();
+      return fail(RSG_ERR_DEVICE, "batch graph: %s", hipGetErrorString(e));
     }
-    b->graph = g;
     HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->graph_layout = b->layout;
     b->graph_dirty = false;

@@ -114,13 +114,17 @@ hipError_t launch_velocity_f64(const VelocityArgs<double>& a, int block, hipStre
 hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStream_t s);
 hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t s);
 hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t s);
-// Batched persistent launch: grid workgroups walk work = {n, 0, (instance, block)...};
-// lds selects LDS staging (max_u: the largest staged set of any instance), cpt the
-// entities per thread (1, 2 or 4).
+// Batched launch: grid workgroups walk work = {n, flags, (instance, block)...}; lds
+// selects LDS staging (max_u: the largest staged set of any instance), cpt the entities
+// per thread (1, 2 or 4).  With graph != nullptr the launch is added to the graph as a
+// kernel node that depends on *node (when not null) and *node becomes the new node;
+// otherwise it is enqueued on s.
 hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* work, int k, int grid, int block,
-                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s);
+                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s, hipGraph_t graph,
+                            hipGraphNode_t* node);
 hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* work, int k, int grid, int block,
-                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s);
+                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s, hipGraph_t graph,
+                            hipGraphNode_t* node);
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s);
 hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nblocks, hipStream_t s);
 

@@ -603,13 +603,28 @@ hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t 
 }
 template <typename T>
 hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* work, int k, int grid, int block, bool lds,
-                        int max_u, int cpt, bool velocity, hipStream_t s) {
+                        int max_u, int cpt, bool velocity, hipStream_t s, hipGraph_t graph,
+                        hipGraphNode_t* node) {
   if (grid <= 0) return hipSuccess;
   if (block < 64 || block > RSG_MAX_BLOCK || (block % 64) != 0) return hipErrorInvalidValue;
   const size_t bytes = lds ? (size_t)(velocity ? 6 : 3) * max_u * sizeof(T) : 0;
   return with_variant(lds, max_u, block, cpt, [&](auto L, auto U, auto C) {
     constexpr bool l = decltype(L)::value;
     constexpr int u = decltype(U)::value, c = decltype(C)::value;
+    if (graph) {
+      const void* fn = velocity ? reinterpret_cast<const void*>(&velocity_batch_kernel<T, l, u, c>)
+                                : reinterpret_cast<const void*>(&stress_batch_kernel<T, l, u, c>);
+      void* args[3] = {(void*)&desc, (void*)&work, (void*)&k};
+      hipKernelNodeParams p{};
+      p.func = const_cast<void*>(fn);
+      p.gridDim = dim3(grid);
+      p.blockDim = dim3(block);
+      p.sharedMemBytes = (unsigned)bytes;
+      p.kernelParams = args;
+      p.extra = nullptr;
+      const hipGraphNode_t prev = *node;
+      return hipGraphAddKernelNode(node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+    }
     if (velocity)
       hipLaunchKernelGGL((velocity_batch_kernel<T, l, u, c>), dim3(grid), dim3(block), bytes, s, desc, work, k);
     else
@@ -619,12 +634,14 @@ hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* work, int k, in
 }
 
 hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* work, int k, int grid, int block,
-                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s) {
-  return launch_batch<double>(d, work, k, grid, block, lds, max_u, cpt, velocity, s);
+                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s, hipGraph_t graph,
+                            hipGraphNode_t* node) {
+  return launch_batch<double>(d, work, k, grid, block, lds, max_u, cpt, velocity, s, graph, node);
 }
 hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* work, int k, int grid, int block,
-                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s) {
-  return launch_batch<float>(d, work, k, grid, block, lds, max_u, cpt, velocity, s);
+                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s, hipGraph_t graph,
+                            hipGraphNode_t* node) {
+  return launch_batch<float>(d, work, k, grid, block, lds, max_u, cpt, velocity, s, graph, node);
 }
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(energy_kernel<double>, dim3(nblocks), dim3(256), 0, s, a);

@@ -1,0 +1,323 @@
+"""Served inspection stream: parametric RSG solves on one GPU with continuous batching.
+
+Requests (a cracked Ti-6Al-4V bracket or a cracked, holed CFRP plate, each with its own
+crack length / depth / orientation, transducer position, mesh size and step count)
+arrive as a seeded Poisson process.  The server
+
+* admits arrived requests in order while their device footprint fits the HBM budget
+  (fields + coefficients + connectivity + LDS plans, estimated from the mesh size before
+  the solve is created) and the resident cell count stays under ``max_resident_cells``;
+* builds each admitted job and creates its solver on host threads while the GPU works;
+* folds ready solves into the one running batch at K-step boundaries (rsg_batch_add),
+  where they share every launch with the solves already resident, and retires each solve
+  when its own step count is reached (its fields stay resident until then);
+* re-runs a seeded fraction of the requests in FP64 (a second batch on its own stream)
+  and states each checked result's relative L2 sensor-trace error against it.
+
+Latency is the time from a request's arrival to its result being on the host.  Nothing
+here is in the reference, which runs one solve per process
+(applications/source/CaseRunners/TransducerModel.cpp); every solve's arithmetic is
+rsg_step's, so a served result is bit-identical to the same request solved alone.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import dataclasses
+import json
+import math
+import sys
+import time
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import cases, rsg
+
+# Device bytes per cell and per node of a solver (fields, coefficients, links, LDS plans,
+# energy weights), from rsg_create's allocations; a 5 % margin and 4 MiB per solve for
+# the amplitude and trace buffers are added on top.
+BYTES_PER_CELL = {rsg.RSG_FP64: 303, rsg.RSG_FP32: 183}
+BYTES_PER_NODE = {rsg.RSG_FP64: 119, rsg.RSG_FP32: 91}
+
+
+def estimate_device_bytes(n_cells: int, n_nodes: int, precision: int) -> int:
+    return int(1.05 * (BYTES_PER_CELL[precision] * n_cells + BYTES_PER_NODE[precision] * n_nodes)) + (4 << 20)
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float  # seconds after the stream starts
+    kind: str  # "bracket" or "plate"
+    cells: int  # target cell count
+    nsteps: int
+    crack_length: float
+    crack_depth: float
+    crack_angle: float
+    xd: tuple  # transducer position on the inspection face, fractions of its extent
+    seed: int
+    spot_check: bool
+
+
+def request_stream(seed=9, n=1000, rate=2.0, cells=(0.5e6, 8.0e6), steps=(1000, 4000), spot_fraction=0.05,
+                   kinds=("bracket", "plate")) -> List[Request]:
+    """Seeded Poisson arrivals at ``rate`` per second; mesh sizes log-uniform in ``cells``,
+    steps uniform in ``steps``, crack length U(0.5, 5) mm, depth U(0.2, 2) mm, orientation
+    U(0, 180) deg, transducer uniform on the inspection face."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        kind = kinds[int(rng.integers(0, len(kinds)))]
+        target = int(math.exp(rng.uniform(math.log(cells[0]), math.log(cells[1]))))
+        out.append(Request(rid=i, arrival=t, kind=kind, cells=target,
+                           nsteps=int(rng.integers(steps[0], steps[1] + 1)),
+                           crack_length=float(rng.uniform(0.5e-3, 5.0e-3)),
+                           crack_depth=float(rng.uniform(0.2e-3, 2.0e-3)),
+                           crack_angle=float(rng.uniform(0.0, 180.0)),
+                           xd=(float(rng.uniform()), float(rng.uniform())),
+                           seed=int(rng.integers(0, 2 ** 31 - 1)), spot_check=bool(rng.uniform() < spot_fraction)))
+    return out
+
+
+def mesh_dims(req: Request):
+    """Grid dimensions for the request's target cell count."""
+    if req.kind == "bracket":  # two legs of leg x w x w cells, leg = 16/3 w as in 512 x 96 x 96
+        w = max(12, int(round((req.cells * 3 / 32) ** (1 / 3))))
+        return dict(leg=max(2 * w, int(round(w * 16 / 3))), width=w, height=w)
+    n = max(24, int(round(math.sqrt(req.cells / 64))))  # plate: n x n x 64 cells
+    return dict(nx=n, ny=n)
+
+
+def build_case(req: Request) -> cases.Case:
+    dx = 5.0e-5
+    d = mesh_dims(req)
+    u, v = req.xd
+    if req.kind == "bracket":
+        L, W = d["leg"] * dx, d["width"] * dx
+        return cases.ti_bracket(seed=req.seed, dx=dx, nsteps=req.nsteps, crack_length=req.crack_length,
+                                crack_depth=req.crack_depth, crack_angle=req.crack_angle,
+                                xd_center=((0.1 + 0.35 * u) * L, (0.2 + 0.6 * v) * W), **d)
+    lx, ly, lz = d["nx"] * dx, d["ny"] * dx, 64 * dx
+    rng = np.random.default_rng(req.seed)
+    crack = cases.Crack((rng.uniform(0.2, 0.8) * lx, rng.uniform(0.2, 0.8) * ly, lz), req.crack_angle,
+                        req.crack_length, min(req.crack_depth, 0.8 * lz))
+    return cases.cfrp_plate(seed=req.seed, nsteps=req.nsteps, renumber=False, crack=crack,
+                            xd_center=((0.25 + 0.5 * u) * lx, (0.25 + 0.5 * v) * ly), **d)
+
+
+def rel_l2_traces(tr, ref):
+    """Per-sensor relative L2 error over all steps and components (sensors whose reference
+    norm is below 1e-6 of the largest are left out)."""
+    num = np.sqrt(((tr - ref) ** 2).sum(axis=(0, 2)))
+    den = np.sqrt((ref ** 2).sum(axis=(0, 2)))
+    keep = den > 1e-6 * max(den.max(), 1e-300)
+    return num[keep] / den[keep]
+
+
+@dataclasses.dataclass
+class _Job:
+    req: Request
+    case: cases.Case
+    solver: rsg.DeviceSolver
+    bytes: int
+    slot: int = -1
+    admitted: float = 0.0
+    ready: float = 0.0
+    started: float = 0.0
+
+
+class Server:
+    """Continuous-batching server over one rsg batch (plus one FP64 batch for spot checks)."""
+
+    def __init__(self, precision=rsg.RSG_FP32, hbm_budget=96 << 30, max_resident_cells=48_000_000,
+                 steps_per_block=50, prep_threads=6, keep_traces=False, log=None):
+        self.precision = precision
+        self.hbm_budget = int(hbm_budget)
+        self.max_resident_cells = int(max_resident_cells)
+        self.K = int(steps_per_block)
+        self.prep_threads = int(prep_threads)
+        self.keep_traces = keep_traces
+        self.log = log or (lambda *a: None)
+
+    def _prepare(self, req: Request, precision: int, case: Optional[cases.Case] = None):
+        t0 = time.perf_counter()
+        if case is None:
+            case = build_case(req)
+        s = rsg.DeviceSolver.from_case(case, precision)
+        return case, s, time.perf_counter() - t0
+
+    def serve(self, requests: List[Request], time_scale: float = 1.0) -> Dict:
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        pool = cf.ThreadPoolExecutor(max_workers=self.prep_threads)
+        main = rsg.Batch(self.precision, steps_per_block=self.K)
+        check = rsg.Batch(rsg.RSG_FP64, steps_per_block=self.K)
+        start = time.perf_counter()
+        now = lambda: time.perf_counter() - start  # noqa: E731
+        nxt = 0  # next request not yet admitted
+        used_bytes = 0
+        resident_cells = 0
+        preps = {}  # future -> (request, bytes, admitted time, is spot check)
+        active: Dict[int, _Job] = {}
+        checking: Dict[int, _Job] = {}
+        results: Dict[int, Dict] = {}
+        spot_errors = []
+        gpu_busy = 0.0
+        prep_seconds = []
+        cell_steps = 0
+        last_log = 0.0
+
+        def admit():
+            nonlocal nxt, used_bytes, resident_cells
+            while nxt < len(reqs) and reqs[nxt].arrival * time_scale <= now():
+                r = reqs[nxt]
+                nc = 2 * r.cells  # upper bound before the mesh exists (holes, rounding)
+                est = estimate_device_bytes(nc, int(1.1 * nc), self.precision)
+                busy = used_bytes > 0 or resident_cells > 0
+                if busy and (used_bytes + est > self.hbm_budget or resident_cells + r.cells > self.max_resident_cells):
+                    return
+                used_bytes += est
+                resident_cells += r.cells
+                preps[pool.submit(self._prepare, r, self.precision)] = (r, est, now(), False)
+                nxt += 1
+
+        def collect(block: bool, timeout: float):
+            nonlocal used_bytes
+            if not preps:
+                return
+            done, _ = cf.wait(list(preps), timeout=timeout if block else 0, return_when=cf.FIRST_COMPLETED)
+            for f in done:
+                r, est, t_adm, is_check = preps.pop(f)
+                case, s, secs = f.result()
+                prep_seconds.append(secs)
+                job = _Job(req=r, case=case, solver=s, bytes=est, admitted=t_adm, ready=now())
+                if is_check:
+                    job.slot = check.add(s, r.nsteps)
+                    checking[job.slot] = job
+                else:
+                    job.slot = main.add(s, r.nsteps)
+                    job.started = now()
+                    active[job.slot] = job
+
+        while nxt < len(reqs) or preps or active or checking:
+            if now() - last_log >= 15.0:
+                last_log = now()
+                self.log("serve: t=%.0f s, %d done, %d active, %d preparing, %d checking, %d waiting"
+                         % (last_log, len(results) - len(checking), len(active), len(preps), len(checking),
+                            len(reqs) - nxt))
+            admit()
+            collect(block=False, timeout=0)
+            if not active and not checking:
+                wait = None
+                if nxt < len(reqs):
+                    wait = max(0.0, reqs[nxt].arrival * time_scale - now())
+                if preps:
+                    collect(block=True, timeout=wait if wait is not None else 1.0)
+                elif wait is not None:
+                    time.sleep(min(wait, 0.05))
+                continue
+            t0 = time.perf_counter()
+            if active:
+                main.run(1)
+            if checking:
+                check.run(1)
+            main.synchronize()
+            check.synchronize()
+            gpu_busy += time.perf_counter() - t0
+            t_done = now()
+            for slot in list(active):
+                job = active[slot]
+                done, total = main.status(slot)
+                if done < total:
+                    continue
+                tr = job.solver.traces()
+                main.remove(slot)
+                del active[slot]
+                r = job.req
+                cell_steps += job.case.n_cells * r.nsteps
+                resident_cells -= r.cells
+                results[r.rid] = dict(rid=r.rid, kind=r.kind, cells=job.case.n_cells, nsteps=r.nsteps,
+                                      arrival=r.arrival * time_scale, admitted=job.admitted, ready=job.ready,
+                                      finished=t_done, latency=t_done - r.arrival * time_scale)
+                if self.keep_traces:
+                    results[r.rid]["traces"] = tr
+                job.solver.close()
+                if r.spot_check:  # the FP64 re-run shares the byte budget; its solve reuses the case
+                    est = estimate_device_bytes(job.case.n_cells, job.case.n_nodes, rsg.RSG_FP64)
+                    used_bytes += est - job.bytes
+                    f = pool.submit(self._prepare, r, rsg.RSG_FP64, job.case)
+                    preps[f] = (r, est, t_done, True)
+                    results[r.rid]["fp32_traces"] = tr
+                else:
+                    used_bytes -= job.bytes
+            for slot in list(checking):
+                job = checking[slot]
+                done, total = check.status(slot)
+                if done < total:
+                    continue
+                tr64 = job.solver.traces()
+                check.remove(slot)
+                del checking[slot]
+                res = results[job.req.rid]
+                err = rel_l2_traces(res.pop("fp32_traces"), tr64)
+                # None when no sensor saw the wave in the FP64 run (nothing to compare)
+                res["spot_check_trace_rel_l2"] = float(err.max()) if err.size else None
+                res["spot_check_sensors"] = int(err.size)
+                if err.size:
+                    spot_errors.append(res["spot_check_trace_rel_l2"])
+                cell_steps += job.case.n_cells * job.req.nsteps
+                used_bytes -= job.bytes
+                job.solver.close()
+        elapsed = now()
+        main.close()
+        check.close()
+        pool.shutdown()
+        lat = np.array([results[r.rid]["latency"] for r in reqs])
+        served_cell_steps = sum(results[r.rid]["cells"] * results[r.rid]["nsteps"] for r in reqs)
+        report = dict(
+            requests=len(reqs), elapsed_s=elapsed, solves_per_hour=len(reqs) / elapsed * 3600.0,
+            latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),
+            latency_mean_s=float(lat.mean()),
+            gcell_steps_per_s=served_cell_steps / elapsed / 1e9,
+            gcell_steps_per_s_with_checks=cell_steps / elapsed / 1e9,
+            gpu_busy_fraction=gpu_busy / elapsed,
+            prep_seconds_mean=float(np.mean(prep_seconds)) if prep_seconds else 0.0,
+            spot_checks=sum(1 for r in reqs if r.spot_check),
+            spot_check_max_trace_rel_l2=max(spot_errors) if spot_errors else None,
+            results=[results[r.rid] for r in reqs])
+        return report
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="Serve a seeded stream of parametric RSG inspection solves")
+    ap.add_argument("--requests", type=int, default=1000)
+    ap.add_argument("--rate", type=float, default=2.0, help="arrivals per second")
+    ap.add_argument("--seed", type=int, default=9)
+    ap.add_argument("--min-cells", type=float, default=0.5e6)
+    ap.add_argument("--max-cells", type=float, default=8.0e6)
+    ap.add_argument("--min-steps", type=int, default=1000)
+    ap.add_argument("--max-steps", type=int, default=4000)
+    ap.add_argument("--spot-fraction", type=float, default=0.05)
+    ap.add_argument("--hbm-budget-gib", type=float, default=96.0)
+    ap.add_argument("--max-resident-cells", type=float, default=48e6)
+    ap.add_argument("--steps-per-block", type=int, default=50)
+    ap.add_argument("--prep-threads", type=int, default=6)
+    ap.add_argument("--out", default=None, help="JSON report path")
+    a = ap.parse_args(argv)
+    reqs = request_stream(seed=a.seed, n=a.requests, rate=a.rate, cells=(a.min_cells, a.max_cells),
+                          steps=(a.min_steps, a.max_steps), spot_fraction=a.spot_fraction)
+    srv = Server(hbm_budget=int(a.hbm_budget_gib * (1 << 30)), max_resident_cells=int(a.max_resident_cells),
+                 steps_per_block=a.steps_per_block, prep_threads=a.prep_threads,
+                 log=lambda *m: print(*m, file=sys.stderr, flush=True))
+    rep = srv.serve(reqs)
+    summary = {k: v for k, v in rep.items() if k != "results"}
+    print(json.dumps(summary), flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(rep, f)
+
+
+if __name__ == "__main__":
+    main()
