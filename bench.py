@@ -32,7 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes (each array element touched once per pass), FP64 data model:
+# Algorithmic bytes (each array element touched once per pass) of the reference's data model
+# (per-entity coefficients, 32-byte connectivity), FP64:
 #   stress kernel per cell: C 21x8 + S read+write 6x8x2 + spacings 3x8 + vertices 8x4
 #                           + V gathered 3x8                                  = 344
 #   velocity kernel per node: V read+write 3x8x2 + dt/rho 8 + spacings 3x8 + cell links
@@ -67,9 +68,10 @@ def run_timed(rsg, case, precision, warmup, steps):
     traces = s.traces()
     V, S = s.get_state()
     dev_bytes = s.device_bytes()
+    tuning = s.tuning()
     s.close()
     return dict(seconds=t1 - t0, vel_ms=vms, str_ms=sms, launches=launches, traces=traces, V=V, S=S,
-                dev_bytes=dev_bytes)
+                dev_bytes=dev_bytes, tuning=tuning)
 
 
 def create_all(rsg, case_list, precision, threads=8):
@@ -195,11 +197,17 @@ def main():
     log("bench: FP64 %.3f Gcell-steps/s, %.4f ms/step; stress %.1f us (%.0f GB/s), velocity %.1f us (%.0f GB/s)"
         % (value, ms_step, str_avg * 1e6, str_gbs, vel_avg * 1e6, vel_gbs))
 
+    # measured HBM bytes per launch of this layout (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "stress_kernel_traffic.json")
-    if os.path.exists(tpath):
+    tpath = os.path.join(ROOT, "profiles", "rsg_kernels_traffic_cfrp_plate.json")
+    if os.path.exists(tpath) and args.seed == 2:
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch_f64")
+            traffic = json.load(f)["hbm_bytes_per_launch"].get("stress_f64")
+    tun = r64["tuning"]
+    layout = {"coefficients": ("class keys: %d cell materials, %d node materials" % (tun["cell_classes"],
+                                                                                   tun["node_classes"])
+                               if tun["cell_classes"] >= 0 else "per-entity arrays"),
+              "links": "16-bit block-local (LDS staging)" if tun["lds_stress"] > 0 else "32-bit global"}
 
     out = {
         "metric": "Gcell-steps/s FP64 vs FP32 @<=1e-4 trace err; served solves/hr + p50/p99 lat",
@@ -222,7 +230,12 @@ def main():
                      "mean_launch_us": round(str_avg * 1e6, 2),
                      "velocity_kernel": {"achieved": round(vel_gbs, 1), "bytes_per_launch": VELOCITY_BYTES["f64"] * nn,
                                          "mean_launch_us": round(vel_avg * 1e6, 2)},
-                     "full_step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+                     "full_step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                     # the layout moves fewer bytes than the reference's data model: measured
+                     # traffic per launch and the bandwidth it achieves
+                     "layout": layout,
+                     "traffic_gbs": round(traffic / str_avg / 1e9, 1) if traffic else None,
+                     "traffic_frac": round(traffic / str_avg / 1e9 / HBM_PEAK_GBS, 4) if traffic else None},
         "device_bytes": r64["dev_bytes"],
     }
 

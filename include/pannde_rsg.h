@@ -125,6 +125,13 @@ typedef struct {
   int32_t tile[3];             /* locality brick in cells along x, y, z (8, 8, 4) */
   int32_t c_layout;            /* 1: C in 64-cell blocks [n/64][21][64], -1: [21][n] (1) */
   int32_t xcd_remap;           /* 1: XCD-contiguous workgroup order, -1: hardware order (1) */
+  int32_t coeff_classes;       /* 1: each entity carries a 4-byte key into small tables of
+                                  materials (a cell's 21 C, a node's dt/rho) and of spacings
+                                  per axis, instead of its own coefficients; used when there
+                                  are at most 1024 materials and 128 spacings along each axis,
+                                  -1: per-entity coefficient arrays (1) */
+  int32_t node_classes;        /* rsg_get_tuning only: node materials in use, -1 = per-node arrays */
+  int32_t cell_classes;        /* rsg_get_tuning only: cell materials in use, -1 = per-cell arrays */
 } rsg_tuning;
 
 /* rsg_create with explicit kernel and layout choices (NULL tuning = defaults). */
@@ -197,7 +204,8 @@ int rsg_last_error(char* buf, size_t len);
  * every instance of a group (a device work table of (instance, block) items and a device
  * table of per-instance arguments), and the K consecutive steps of a block are captured
  * once in a HIP graph.  Instances are grouped so that a group's state arrays (V and S)
- * fit a byte budget (by default 128 MiB, half the MI355X's 256 MiB last-level cache);
+ * fit a byte budget (by default 256 MiB in FP32 and 192 MiB in FP64, sized to the MI355X's
+ * 256 MiB last-level cache);
  * each group takes all K steps of a block before the next group starts, so the state one
  * kernel writes is still cached when the other reads it.  Instances join and leave between K-step blocks; each
  * takes exactly its own number of steps.  No counterpart in the reference, which runs
@@ -226,7 +234,8 @@ int rsg_batch_remove(rsg_batch b, int32_t slot);
 int rsg_batch_run(rsg_batch b, int32_t nblocks, int32_t* unfinished);
 
 /* Scheduling options (results do not depend on them):
- *   RSG_BATCH_GROUP_BYTES  state bytes per group (>= 0; 0: all instances in one group)
+ *   RSG_BATCH_GROUP_BYTES  state bytes per group (>= 0; 0: all instances in one group;
+ *                          default 256 MiB in FP32, 192 MiB in FP64)
  *   RSG_BATCH_XCD_RUN      work-item order across the 8 XCDs: -1 table order, 0 one
  *                          contiguous eighth per XCD, n > 0 runs of n items per XCD in turn
  *                          (default 64) */

@@ -26,6 +26,8 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/pannde_rsg.h"
@@ -133,6 +135,7 @@ struct rsg_instance {
   int xcd = 1;                    // XCD-contiguous workgroup remap
   int lds_v = 1, lds_s = 1;       // stage gathers through LDS when the block sets fit
   int c_blocked = 1;              // C as [n/64][21][64] instead of [21][n]
+  int classes = 1;                // coefficient class tables when the rows repeat
   int tile[3] = {8, 8, 4};        // locality brick (x, y, z) in cells
 
   // permutations: internal index -> caller index, and the inverse
@@ -153,6 +156,16 @@ struct rsg_instance {
   void* dc[3] = {nullptr, nullptr, nullptr};
   int32_t* ncell = nullptr;
   int32_t* cnode = nullptr;
+  // coefficient class tables (null keys: per-entity arrays above)
+  uint32_t* nkey = nullptr;
+  void* nmat = nullptr;   // [materials] dt/rho
+  void* naxis = nullptr;  // [3][kAxisClasses] node spacings
+  int32_t n_nmat = -1;
+  uint32_t* ckey = nullptr;
+  void* cmat = nullptr;   // [materials][kMatRow] C
+  void* caxis = nullptr;  // [3][kAxisClasses] cell spacings
+  int32_t n_cmat = -1;
+  std::vector<uint32_t> nkey_h, ckey_h;  // keys without the special flag, internal order
   // LDS staging plans (null lconn: direct gathers)
   int32_t* v_uoff = nullptr;
   int32_t* v_ulist = nullptr;
@@ -175,6 +188,7 @@ struct rsg_instance {
   DevBuf<int32_t> sp_node_sens;
   DevBuf<int32_t> node_terms;
   DevBuf<char> sp_cell_val;   // T[nsp]
+  DevBuf<int32_t> sp_node_key, sp_cell_key;  // full key of each special entity (class tables)
   DevBuf<int32_t> sp_cell_off;
   DevBuf<int32_t> cell_terms;
   DevBuf<double> amp[2];
@@ -358,6 +372,82 @@ int upload_real(rsg_instance* h, void** dst, const std::vector<double>& src) {
   return upload(h, dst, f);
 }
 
+// Assigns rows of W values (in the handle's precision) to classes of equal rows, in
+// order of first appearance; add() returns -1 once more than max_classes are needed.
+template <typename T, int W>
+struct Classifier {
+  size_t max_classes;
+  std::vector<T> table;
+  std::unordered_map<uint64_t, std::vector<int32_t>> index;
+  int32_t last = -1;
+  explicit Classifier(size_t m) : max_classes(m) {}
+  int32_t add(const T (&row)[W]) {
+    if (last >= 0 && memcmp(row, &table[(size_t)last * W], sizeof(row)) == 0) return last;
+    uint64_t hsh = 1469598103934665603ull;  // FNV-1a over the row's bytes
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(row);
+    for (size_t i = 0; i < sizeof(row); i++) hsh = (hsh ^ b[i]) * 1099511628211ull;
+    auto& list = index[hsh];
+    for (int32_t c : list)
+      if (memcmp(row, &table[(size_t)c * W], sizeof(row)) == 0) return last = c;
+    const size_t n = table.size() / W;
+    if (n >= max_classes) return -1;
+    table.insert(table.end(), row, row + W);
+    list.push_back((int32_t)n);
+    return last = (int32_t)n;
+  }
+};
+
+// Keys of n entities from their material row m (W values) and spacings sp (3 values),
+// both in the handle's precision: material classes and per-axis spacing classes in order
+// of first appearance.  On success uploads the material table (rows of ROW values) and
+// the [3][kAxisClasses] spacing table and sets *materials; leaves it at -1 when the mesh
+// has more than kMaxMaterials materials or kAxisClasses spacings along an axis.
+template <typename T, int W, int ROW, typename F>
+int make_keys(rsg_instance* h, int32_t n, F&& entity, std::vector<uint32_t>& keys, void** mat_table,
+              void** axis_table, int32_t* materials) {
+  *materials = -1;
+  Classifier<T, W> mats(rsg::kMaxMaterials);
+  std::unordered_map<uint64_t, int> ax[3];
+  std::vector<T> axv[3];
+  keys.resize(n);
+  T m[W], sp[3];
+  for (int32_t i = 0; i < n; i++) {
+    entity(i, m, sp);
+    const int32_t mc = mats.add(m);
+    if (mc < 0) {
+      keys.clear();
+      return RSG_OK;
+    }
+    int ic[3];
+    for (int d = 0; d < 3; d++) {
+      uint64_t bits = 0;
+      memcpy(&bits, &sp[d], sizeof(T));
+      auto it = ax[d].find(bits);
+      if (it == ax[d].end()) {
+        if ((int)axv[d].size() >= rsg::kAxisClasses) {
+          keys.clear();
+          return RSG_OK;
+        }
+        it = ax[d].emplace(bits, (int)axv[d].size()).first;
+        axv[d].push_back(sp[d]);
+      }
+      ic[d] = it->second;
+    }
+    keys[i] = rsg::make_key(mc, ic[0], ic[1], ic[2]);
+  }
+  const size_t nm = mats.table.size() / W;
+  std::vector<T> mt(nm * ROW, T(0)), at((size_t)3 * rsg::kAxisClasses, T(0));
+  for (size_t c = 0; c < nm; c++)
+    for (int j = 0; j < W; j++) mt[c * ROW + j] = mats.table[c * W + j];
+  for (int d = 0; d < 3; d++)
+    for (size_t k = 0; k < axv[d].size(); k++) at[(size_t)d * rsg::kAxisClasses + k] = axv[d][k];
+  int rc;
+  if ((rc = upload(h, mat_table, mt))) return rc;
+  if ((rc = upload(h, axis_table, at))) return rc;
+  *materials = (int32_t)nm;
+  return RSG_OK;
+}
+
 // Uploads src in the handle's precision with the entries listed in boxes = {(index,
 // special id)} replaced by NaN boxes written directly in the target format.
 int copy_boxed(rsg_instance* h, void* dst, const std::vector<double>& src,
@@ -457,6 +547,23 @@ int build_tables(rsg_instance* h) {
   }
   h->n_sp_node = (int32_t)sp_idt.size();
   h->n_sp_cell = (int32_t)sp_dcx.size();
+  // class-table form: keys carry the special flag; the special tables carry the class
+  std::vector<int32_t> sp_nkey, sp_ckey;
+  std::vector<uint32_t> nkeys, ckeys;
+  if (h->nkey) {
+    nkeys = h->nkey_h;
+    for (auto& b : node_boxes) {
+      sp_nkey.push_back((int32_t)h->nkey_h[b.first]);
+      nkeys[b.first] = rsg::kSpecialKey | (uint32_t)b.second;
+    }
+  }
+  if (h->ckey) {
+    ckeys = h->ckey_h;
+    for (auto& b : cell_boxes) {
+      sp_ckey.push_back((int32_t)h->ckey_h[b.first]);
+      ckeys[b.first] = rsg::kSpecialKey | (uint32_t)b.second;
+    }
+  }
 
   auto put_real = [&](DevBuf<char>& buf, const std::vector<double>& v) -> int {
     int rc = buf.alloc(std::max<size_t>(1, v.size()) * h->elt());
@@ -484,8 +591,16 @@ int build_tables(rsg_instance* h) {
   if ((rc = put_real(h->sp_cell_val, sp_dcx))) return rc;
   if ((rc = put_int(h->sp_cell_off, csp_off))) return rc;
   if ((rc = put_int(h->cell_terms, cterms))) return rc;
-  if ((rc = copy_boxed(h, h->idt, idt, node_boxes))) return rc;
-  if ((rc = copy_boxed(h, h->dc[0], dcx, cell_boxes))) return rc;
+  if ((rc = put_int(h->sp_node_key, sp_nkey))) return rc;
+  if ((rc = put_int(h->sp_cell_key, sp_ckey))) return rc;
+  if (h->nkey)
+    HIP_TRY(hipMemcpy(h->nkey, nkeys.data(), nkeys.size() * 4, hipMemcpyHostToDevice));
+  else if ((rc = copy_boxed(h, h->idt, idt, node_boxes)))
+    return rc;
+  if (h->ckey)
+    HIP_TRY(hipMemcpy(h->ckey, ckeys.data(), ckeys.size() * 4, hipMemcpyHostToDevice));
+  else if ((rc = copy_boxed(h, h->dc[0], dcx, cell_boxes)))
+    return rc;
   // amplitude staging for kBlockSteps steps
   const size_t nch = std::max<size_t>(1, h->channels.size());
   for (int b = 0; b < 2; b++) {
@@ -541,6 +656,10 @@ void fill_args(rsg_instance* h, rsg::VelocityArgs<T>& va, rsg::StressArgs<T>& sa
   va.sp_off = h->sp_node_off.p;
   va.sp_sens = h->sp_node_sens.p;
   va.term = h->node_terms.p;
+  va.key = h->nkey;
+  va.cls = (const T*)h->nmat;
+  va.axis = (const T*)h->naxis;
+  va.sp_key = (const uint32_t*)h->sp_node_key.p;
   sa.S0 = (T*)h->S[0];
   sa.S1 = (T*)h->S[1];
   sa.S2 = (T*)h->S[2];
@@ -567,6 +686,10 @@ void fill_args(rsg_instance* h, rsg::VelocityArgs<T>& va, rsg::StressArgs<T>& sa
   sa.sp_dcx = (const T*)h->sp_cell_val.p;
   sa.sp_off = h->sp_cell_off.p;
   sa.term = h->cell_terms.p;
+  sa.key = h->ckey;
+  sa.cls = (const T*)h->cmat;
+  sa.axis = (const T*)h->caxis;
+  sa.sp_key = (const uint32_t*)h->sp_cell_key.p;
 }
 
 int ensure_events(rsg_instance* h, size_t n) {
@@ -665,6 +788,9 @@ int rsg_get_tuning(rsg_handle h, rsg_tuning* t) {
   for (int d = 0; d < 3; d++) t->tile[d] = h->tile[d];
   t->c_layout = h->c_blocked ? 1 : -1;
   t->xcd_remap = h->xcd ? 1 : -1;
+  t->coeff_classes = h->classes ? 1 : -1;
+  t->node_classes = h->n_nmat;
+  t->cell_classes = h->n_cmat;
   return RSG_OK;
 }
 
@@ -700,6 +826,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if (t.lds_stress) h->lds_s = t.lds_stress > 0;
     if (t.c_layout) h->c_blocked = t.c_layout > 0;
     if (t.xcd_remap) h->xcd = t.xcd_remap > 0;
+    if (t.coeff_classes) h->classes = t.coeff_classes > 0;
     for (int d = 0; d < 3; d++) {
       if (t.tile[d] < 0) return fail(RSG_ERR_ARG, "tile[%d] = %d is negative", d, t.tile[d]);
       if (t.tile[d]) h->tile[d] = t.tile[d];
@@ -814,9 +941,24 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
       }
     }
     h->idt_h = idt;
-    if ((rc = upload_real(h.get(), &h->idt, idt))) return rc;
-    for (int d = 0; d < 3; d++)
-      if ((rc = upload_real(h.get(), &h->dn[d], dn[d]))) return rc;
+    if (h->classes) {  // material: dt/rho; spacings as the per-node arrays hold them
+      auto entity = [&](int32_t i, auto& m, auto& sp) {
+        using T = std::remove_reference_t<decltype(m[0])>;
+        m[0] = (T)idt[i];
+        for (int d = 0; d < 3; d++) sp[d] = (T)dn[d][i];
+      };
+      rc = precision == RSG_FP64
+               ? make_keys<double, 1, 1>(h.get(), nn, entity, h->nkey_h, &h->nmat, &h->naxis, &h->n_nmat)
+               : make_keys<float, 1, 1>(h.get(), nn, entity, h->nkey_h, &h->nmat, &h->naxis, &h->n_nmat);
+      if (rc) return rc;
+    }
+    if (h->n_nmat >= 0) {
+      if ((rc = h->dalloc((void**)&h->nkey, (size_t)nn * 4))) return rc;
+    } else {
+      if ((rc = upload_real(h.get(), &h->idt, idt))) return rc;
+      for (int d = 0; d < 3; d++)
+        if ((rc = upload_real(h.get(), &h->dn[d], dn[d]))) return rc;
+    }
     void* p;
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->ncell = (int32_t*)p;
@@ -841,23 +983,43 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     std::vector<int32_t> conn((size_t)nc * 8);
     const int64_t ncp = ((int64_t)nc + 63) / 64 * 64;
     h->ldc = h->c_blocked ? 0 : ncp;
-    std::vector<double> C((size_t)21 * ncp, 0.0);
     for (int32_t i = 0; i < nc; i++) {
       const int32_t c = h->cell_of[i];
       for (int d = 0; d < 3; d++)
         dc[d][i] = precision == RSG_FP64 ? dcell[d][c] : 0.25 / dcell[d][c];
       w[i] = dcell[0][c] * dcell[1][c] * dcell[2][c];
       for (int k = 0; k < 8; k++) conn[(size_t)i * 8 + k] = h->node_at[h->cells[(size_t)c * 8 + k]];
-      for (int j = 0; j < 21; j++) {
-        const double cij = field_at_cell(model->cij[j], h->cells.data(), c);
-        const size_t idx = h->c_blocked ? ((size_t)(i >> 6) * 21 + j) * 64 + (i & 63) : (size_t)j * ncp + i;
-        C[idx] = precision == RSG_FP64 ? cij : h->dt * cij;
-      }
     }
     h->dcx_h = dc[0];
-    for (int d = 0; d < 3; d++)
-      if ((rc = upload_real(h.get(), &h->dc[d], dc[d]))) return rc;
-    if ((rc = upload_real(h.get(), &h->C, C))) return rc;
+    // C entry j of internal cell i as the kernels read it: FP64 C_IJ, FP32 dt*C_IJ
+    auto c_entry = [&](int32_t i, int j) {
+      const double cij = field_at_cell(model->cij[j], h->cells.data(), h->cell_of[i]);
+      return precision == RSG_FP64 ? cij : h->dt * cij;
+    };
+    if (h->classes) {  // material: the 21 C; spacings as the per-cell arrays hold them
+      auto entity = [&](int32_t i, auto& m, auto& sp) {
+        using T = std::remove_reference_t<decltype(m[0])>;
+        for (int j = 0; j < 21; j++) m[j] = (T)c_entry(i, j);
+        for (int d = 0; d < 3; d++) sp[d] = (T)dc[d][i];
+      };
+      rc = precision == RSG_FP64
+               ? make_keys<double, 21, rsg::kMatRow>(h.get(), nc, entity, h->ckey_h, &h->cmat, &h->caxis, &h->n_cmat)
+               : make_keys<float, 21, rsg::kMatRow>(h.get(), nc, entity, h->ckey_h, &h->cmat, &h->caxis, &h->n_cmat);
+      if (rc) return rc;
+    }
+    if (h->n_cmat >= 0) {
+      if ((rc = h->dalloc((void**)&h->ckey, (size_t)nc * 4))) return rc;
+    } else {
+      std::vector<double> C((size_t)21 * ncp, 0.0);
+      for (int32_t i = 0; i < nc; i++)
+        for (int j = 0; j < 21; j++) {
+          const size_t idx = h->c_blocked ? ((size_t)(i >> 6) * 21 + j) * 64 + (i & 63) : (size_t)j * ncp + i;
+          C[idx] = c_entry(i, j);
+        }
+      for (int d = 0; d < 3; d++)
+        if ((rc = upload_real(h.get(), &h->dc[d], dc[d]))) return rc;
+      if ((rc = upload_real(h.get(), &h->C, C))) return rc;
+    }
     void* p;
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->cnode = (int32_t*)p;
@@ -878,14 +1040,14 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   // zero state (TransducerModel.cpp:150-159)
   for (int d = 0; d < 3; d++) {
     if ((rc = h->dalloc(&h->V[d], (size_t)nn * h->elt()))) return rc;
-    HIP_TRY(hipMemset(h->V[d], 0, (size_t)nn * h->elt()));
+    HIP_TRY(hipMemsetAsync(h->V[d], 0, (size_t)nn * h->elt(), h->stream));
   }
   for (int s = 0; s < 6; s++) {
     if ((rc = h->dalloc(&h->S[s], (size_t)nc * h->elt()))) return rc;
-    HIP_TRY(hipMemset(h->S[s], 0, (size_t)nc * h->elt()));
+    HIP_TRY(hipMemsetAsync(h->S[s], 0, (size_t)nc * h->elt(), h->stream));
   }
   if ((rc = h->partial.alloc(2 * 1024))) return rc;
-  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipStreamSynchronize(h->stream));  // this handle's work only (others may be running)
   *out = h.release();
   return RSG_OK;
 }
@@ -1216,8 +1378,10 @@ struct rsg_batch_s {
   bool dense = false;   // one workgroup per work item
   bool use_graph = true;
   // instances are scheduled in groups whose state arrays (V and S) fit this many bytes;
-  // each group takes all K steps of a block before the next starts (0: one group)
-  int64_t group_bytes = (int64_t)128 << 20;
+  // each group takes all K steps of a block before the next starts (0: one group).  Set at
+  // creation: 256 MiB for FP32, 192 MiB for FP64 (measured optima on 64 instances of
+  // 32^3..128^3 cells)
+  int64_t group_bytes = (int64_t)192 << 20;
   // work-table order across XCDs: -1 identity, 0 one contiguous eighth per XCD, n > 0 runs
   // of n consecutive items per XCD in turn
   int32_t xcd_run = 64;
@@ -1247,8 +1411,10 @@ struct rsg_batch_s {
   struct Launch {
     int64_t off_v, off_s;
     int32_t grid_v, grid_s;
+    int32_t km_v, km_s;  // coefficient forms (0 arrays, 1 keys, 2 mixed)
     bool operator==(const Launch& o) const {
-      return off_v == o.off_v && off_s == o.off_s && grid_v == o.grid_v && grid_s == o.grid_s;
+      return off_v == o.off_v && off_s == o.off_s && grid_v == o.grid_v && grid_s == o.grid_s &&
+             km_v == o.km_v && km_s == o.km_s;
     }
   };
   std::vector<Launch> layout, graph_layout;
@@ -1350,16 +1516,16 @@ int batch_launch_block(rsg_batch_s* b) {
         hipError_t e;
         if (sizeof(T) == 8) {
           e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv + g.off_v, k, g.grid_v, b->block_v,
-                                    b->lds_v, b->max_u_v, b->cpt_v, true, b->stream, graph, node);
+                                    b->lds_v, b->max_u_v, b->cpt_v, g.km_v, true, b->stream, graph, node);
           if (e != hipSuccess) return e;
           e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws + g.off_s, k, g.grid_s, b->block_s,
-                                    b->lds_s, b->max_u_s, b->cpt_s, false, b->stream, graph, node);
+                                    b->lds_s, b->max_u_s, b->cpt_s, g.km_s, false, b->stream, graph, node);
         } else {
           e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv + g.off_v, k, g.grid_v, b->block_v,
-                                    b->lds_v, b->max_u_v, b->cpt_v, true, b->stream, graph, node);
+                                    b->lds_v, b->max_u_v, b->cpt_v, g.km_v, true, b->stream, graph, node);
           if (e != hipSuccess) return e;
           e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws + g.off_s, k, g.grid_s, b->block_s,
-                                    b->lds_s, b->max_u_s, b->cpt_s, false, b->stream, graph, node);
+                                    b->lds_s, b->max_u_s, b->cpt_s, g.km_s, false, b->stream, graph, node);
         }
         if (e != hipSuccess) return e;
       }
@@ -1449,6 +1615,15 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
   // groups of consecutive active instances, each within the state budget (an instance
   // larger than the budget forms a group of its own)
   const int32_t flags = b->xcd_run < 0 ? 0 : 1 | (b->xcd_run << 8);
+  // coefficient form of each kernel over the active instances: keys, arrays or both
+  int form_v = 0, form_s = 0;  // bit 0: some instance uses arrays, bit 1: some uses keys
+  for (size_t i = 0; i < b->slots.size(); i++)
+    if (kmax[i]) {
+      form_v |= b->slots[i].h->nkey ? 2 : 1;
+      form_s |= b->slots[i].h->ckey ? 2 : 1;
+    }
+  const int32_t km_v = form_v == 3 ? 2 : form_v == 2 ? 1 : 0;
+  const int32_t km_s = form_s == 3 ? 2 : form_s == 2 ? 1 : 0;
   const int32_t cv = b->block_v * b->cpt_v, cs = b->block_s * b->cpt_s;
   b->layout.clear();
   size_t ov = 0, os = 0;
@@ -1491,7 +1666,7 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
     }
     const int32_t qv = (nv + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
     const int32_t qs = (ns + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
-    b->layout.push_back({(int64_t)ov, (int64_t)os, b->dense ? qv : b->grid, b->dense ? qs : b->grid});
+    b->layout.push_back({(int64_t)ov, (int64_t)os, b->dense ? qv : b->grid, b->dense ? qs : b->grid, km_v, km_s});
     ov += table_words(nv);
     os += table_words(ns);
   }
@@ -1533,6 +1708,7 @@ int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, i
   std::unique_ptr<rsg_batch_s> b(new (std::nothrow) rsg_batch_s());
   if (!b) return fail(RSG_ERR_NOMEM, "host allocation failed");
   b->precision = precision;
+  b->group_bytes = (int64_t)(precision == RSG_FP32 ? 256 : 192) << 20;
   b->K = steps_per_block ? steps_per_block : 50;
   b->use_graph = use_graph != 0;
   if (grid > 0)

@@ -11,10 +11,28 @@ namespace rsg {
 
 // Kernel argument blocks.  All arrays are in the library's internal (locality) order.
 //
-// Source and sensor entities are flagged without any extra per-entity array: a node's
-// dt/rho (idt) or a cell's x-spacing (dcx) is replaced by a quiet NaN whose payload is
-// the entity's index into the small "special" tables, which hold the real value plus
-// the term list.  Ordinary entities therefore cost no extra HBM bytes.
+// Coefficients come in one of two forms per kernel:
+//  * class tables (key != nullptr): each entity holds a 32-bit key made of a material
+//    class (bits 21..30: the cell's row of 21 C, or the node's dt/rho) and one spacing
+//    class per axis (7 bits each: x at bit 14, y at bit 7, z at bit 0).  The material rows
+//    and the three spacing tables are small and read through the caches; a wavefront's
+//    cells usually share one material row, so those loads coalesce.  A key with bit 31 set
+//    flags a special entity (source or sensor): its low bits index the special tables,
+//    which hold the entity's full key;
+//  * per-entity arrays (key == nullptr), for meshes with more than 1024 materials or 128
+//    spacings along an axis: source and sensor entities are flagged without any extra
+//    array, a node's dt/rho (idt) or a cell's x-spacing (dcx) being replaced by a quiet NaN
+//    whose payload is the special index, and the special tables hold the real value.
+// Either way ordinary entities cost no extra HBM bytes for the flag.
+constexpr uint32_t kSpecialKey = 0x80000000u;
+constexpr int kMatShift = 21;
+constexpr int kMaxMaterials = 1024;
+constexpr int kAxisClasses = 128;  // spacing classes per axis; tables are [3][kAxisClasses]
+constexpr int kMatRow = 24;        // cell material row: the 21 C as the C array holds them, padded
+
+inline uint32_t make_key(int mat, int ix, int iy, int iz) {
+  return ((uint32_t)mat << kMatShift) | ((uint32_t)ix << 14) | ((uint32_t)iy << 7) | (uint32_t)iz;
+}
 template <typename T>
 struct VelocityArgs {
   T* V0;
@@ -46,6 +64,11 @@ struct VelocityArgs {
   const int32_t* term;     // channel id per term
   const double* amp;       // [n_channels] amplitudes of this step
   double* trace;           // [n_sensors][3] row of this step
+  // class tables (key != nullptr: idt, dnx..dnz and sp_idt are not used)
+  const uint32_t* key;     // [n] material/spacing key, or kSpecialKey | special index
+  const T* cls;            // [materials] dt/rho
+  const T* axis;           // [3][kAxisClasses] node spacings as dnx..dnz hold them
+  const uint32_t* sp_key;  // [nsp] key of each special node
 };
 
 template <typename T>
@@ -78,6 +101,11 @@ struct StressArgs {
   const int32_t* sp_off;  // [nsp][7] term ranges for Voigt components 0..5
   const int32_t* term;
   const double* amp;
+  // class tables (key != nullptr: C, dcx..dcz and sp_dcx are not used)
+  const uint32_t* key;     // [n] material/spacing key, or kSpecialKey | special index
+  const T* cls;            // [materials][kMatRow]
+  const T* axis;           // [3][kAxisClasses] cell spacings as dcx..dcz hold them
+  const uint32_t* sp_key;  // [nsp] key of each special cell
 };
 
 // Energy reduction arguments (FP64 accumulation, one partial per block).
@@ -116,14 +144,15 @@ hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t
 hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t s);
 // Batched launch: grid workgroups walk work = {n, flags, (instance, block)...}; lds
 // selects LDS staging (max_u: the largest staged set of any instance), cpt the entities
-// per thread (1, 2 or 4).  With graph != nullptr the launch is added to the graph as a
+// per thread (1, 2 or 4), km the coefficient form (0 per-entity arrays, 1 class-table
+// keys, 2 per instance).  With graph != nullptr the launch is added to the graph as a
 // kernel node that depends on *node (when not null) and *node becomes the new node;
 // otherwise it is enqueued on s.
 hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* work, int k, int grid, int block,
-                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s, hipGraph_t graph,
+                            bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node);
 hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* work, int k, int grid, int block,
-                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s, hipGraph_t graph,
+                            bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node);
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s);
 hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nblocks, hipStream_t s);

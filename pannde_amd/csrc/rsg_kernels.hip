@@ -130,6 +130,59 @@ __device__ __forceinline__ void load8_nodes(const T* __restrict__ f, const int (
 }
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// One material row (16-byte aligned) as 16-byte loads through the caches: lanes of a
+// wavefront that share a material read the same addresses, which coalesce.
+template <int W>
+__device__ __forceinline__ void load_row(const double* __restrict__ p, double (&out)[W]) {
+  static_assert(W % 2 == 0, "row width");
+  const d2v* q = reinterpret_cast<const d2v*>(p);
+#pragma unroll
+  for (int i = 0; i < W / 2; i++) {
+    const d2v v = q[i];
+    out[2 * i] = v.x;
+    out[2 * i + 1] = v.y;
+  }
+}
+template <int W>
+__device__ __forceinline__ void load_row(const float* __restrict__ p, float (&out)[W]) {
+  static_assert(W % 4 == 0, "row width");
+  const f4v* q = reinterpret_cast<const f4v*>(p);
+#pragma unroll
+  for (int i = 0; i < W / 4; i++) {
+    const f4v v = q[i];
+    out[4 * i] = v.x;
+    out[4 * i + 1] = v.y;
+    out[4 * i + 2] = v.z;
+    out[4 * i + 3] = v.w;
+  }
+}
+
+// Coefficient form of a kernel instance: KM = 0 per-entity arrays, 1 class-table keys,
+// 2 either, chosen per instance by a.key (batches that mix the two forms).
+template <int KM, typename A>
+__device__ __forceinline__ bool keyed(const A& a) {
+  return KM == 1 || (KM == 2 && a.key != nullptr);
+}
+
+// Material/spacing key of an entity, and its special index (or -1).
+template <typename A>
+__device__ __forceinline__ uint32_t resolve_key(const A& a, uint32_t key, int& sp) {
+  if (key & kSpecialKey) {
+    sp = (int)(key & ~kSpecialKey);
+    return a.sp_key[sp];
+  }
+  sp = -1;
+  return key;
+}
+template <typename T>
+__device__ __forceinline__ void axis_spacings(const T* __restrict__ axis, uint32_t k, T& sx, T& sy, T& sz) {
+  sx = axis[(k >> 14) & (kAxisClasses - 1)];
+  sy = axis[kAxisClasses + ((k >> 7) & (kAxisClasses - 1))];
+  sz = axis[2 * kAxisClasses + (k & (kAxisClasses - 1))];
+}
 
 // 8 x int32 of one entity as two 16-byte loads; read once per step, so non-temporal.
 __device__ __forceinline__ void load_conn(const int32_t* __restrict__ conn, int i, int (&c)[8]) {
@@ -194,30 +247,54 @@ __device__ __forceinline__ void stage(T* lds, int max_u, const int32_t* __restri
 template <typename T>
 struct VelIn {
   int c[8];
+  uint32_t key;
   T idt, sx, sy, sz, v0, v1, v2;
 };
 
-template <typename T, bool LDS>
+template <typename T, bool LDS, int KM>
 __device__ __forceinline__ void velocity_load(const VelocityArgs<T>& a, int n, VelIn<T>& in) {
   if constexpr (LDS)
     load_lconn(a.lconn, n, in.c);
   else
     load_conn(a.ncell, n, in.c);
-  in.idt = a.idt[n];
-  in.sx = a.dnx[n];
-  in.sy = a.dny[n];
-  in.sz = a.dnz[n];
+  if (keyed<KM>(a)) {
+    in.key = __builtin_nontemporal_load(a.key + n);
+    in.idt = in.sx = in.sy = in.sz = T(0);
+  } else {
+    in.key = 0;
+    in.idt = a.idt[n];
+    in.sx = a.dnx[n];
+    in.sy = a.dny[n];
+    in.sz = a.dnz[n];
+  }
   in.v0 = a.V0[n];
   in.v1 = a.V1[n];
   in.v2 = a.V2[n];
 }
 
 // V += dt/rho * (div S + f) at node n, S gathered from S0..S5 (LDS or HBM).
-template <typename T>
+template <typename T, int KM>
 __device__ __forceinline__ void velocity_apply(const VelocityArgs<T>& a, int n, const VelIn<T>& in,
                                                const T* S0, const T* S1, const T* S2, const T* S3,
                                                const T* S4, const T* S5) {
-  const T sx = in.sx, sy = in.sy, sz = in.sz;
+  // coefficients: the node's key into the class tables, or the per-node arrays (special
+  // nodes NaN-boxed)
+  T idt, sx, sy, sz;
+  int sp = -1;
+  if (keyed<KM>(a)) {
+    const uint32_t k = resolve_key(a, in.key, sp);
+    idt = a.cls[k >> kMatShift];
+    axis_spacings(a.axis, k, sx, sy, sz);
+  } else {
+    idt = in.idt;
+    sx = in.sx;
+    sy = in.sy;
+    sz = in.sz;
+    if (is_boxed(idt)) {
+      sp = unbox(idt);
+      idt = a.sp_idt[sp];
+    }
+  }
   T f[8];
   load8_cells(S0, in.c, f);
   const T dx_sxx = dscale(diffsum<0>(f), sx);
@@ -239,11 +316,7 @@ __device__ __forceinline__ void velocity_apply(const VelocityArgs<T>& a, int n, 
   T r1 = (dx_sxy + dy_syy) + dz_syz;
   T r2 = (dx_sxz + dy_syz) + dz_szz;
 
-  T idt = in.idt;
-  int sp = -1;
-  if (is_boxed(idt)) {
-    sp = unbox(idt);
-    idt = a.sp_idt[sp];
+  if (sp >= 0) {
     const int* off = a.sp_off + 4 * (size_t)sp;
     double fsum[3];
 #pragma unroll
@@ -276,11 +349,11 @@ __device__ __forceinline__ void velocity_apply(const VelocityArgs<T>& a, int n, 
 // (CPT * blockDim.x nodes, thread t taking nodes t, t + blockDim.x, ...).  With LDS the
 // chunk's unique cells are staged first; the first node's streaming operands are
 // requested before the staging so their latency overlaps it.
-template <typename T, bool LDS, int U, int CPT>
+template <typename T, bool LDS, int U, int CPT, int KM>
 __device__ __forceinline__ void velocity_body(const VelocityArgs<T>& a, int b, unsigned char* smem) {
   const int base = b * CPT * (int)blockDim.x + (int)threadIdx.x;
   VelIn<T> in;
-  velocity_load<T, LDS>(a, base < a.n ? base : a.n - 1, in);  // tail threads load a valid node
+  velocity_load<T, LDS, KM>(a, base < a.n ? base : a.n - 1, in);  // tail threads load a valid node
   const T* S0 = a.S0;
   const T* S1 = a.S1;
   const T* S2 = a.S2;
@@ -299,13 +372,13 @@ __device__ __forceinline__ void velocity_body(const VelocityArgs<T>& a, int b, u
     S5 = lds + 5 * a.max_u;
   }
   if (base >= a.n) return;
-  velocity_apply(a, base, in, S0, S1, S2, S3, S4, S5);
+  velocity_apply<T, KM>(a, base, in, S0, S1, S2, S3, S4, S5);
 #pragma unroll 1
   for (int j = 1; j < CPT; j++) {
     const int n = base + j * (int)blockDim.x;
     if (n >= a.n) break;
-    velocity_load<T, LDS>(a, n, in);
-    velocity_apply(a, n, in, S0, S1, S2, S3, S4, S5);
+    velocity_load<T, LDS, KM>(a, n, in);
+    velocity_apply<T, KM>(a, n, in, S0, S1, S2, S3, S4, S5);
   }
 }
 
@@ -327,22 +400,31 @@ __host__ __device__ constexpr int tri(int i, int j) {
 template <typename T>
 struct StrIn {
   int c[8];
+  uint32_t key;
   T sx, sy, sz;
   T cc[21];
   T s[6];
 };
 
-template <typename T, bool LDS>
+template <typename T, bool LDS, int KM>
 __device__ __forceinline__ void stress_load(const StressArgs<T>& a, int e, StrIn<T>& in) {
   if constexpr (LDS)
     load_lconn(a.lconn, e, in.c);
   else
     load_conn(a.cnode, e, in.c);
-  in.sx = a.dcx[e];
-  in.sy = a.dcy[e];
-  in.sz = a.dcz[e];
+  if (keyed<KM>(a)) {
+    in.key = __builtin_nontemporal_load(a.key + e);
+    in.sx = in.sy = in.sz = T(0);
 #pragma unroll
-  for (int j = 0; j < 21; j++) in.cc[j] = cload(a.C, a.ldc, j, e);
+    for (int j = 0; j < 21; j++) in.cc[j] = T(0);
+  } else {
+    in.key = 0;
+    in.sx = a.dcx[e];
+    in.sy = a.dcy[e];
+    in.sz = a.dcz[e];
+#pragma unroll
+    for (int j = 0; j < 21; j++) in.cc[j] = cload(a.C, a.ldc, j, e);
+  }
   in.s[0] = a.S0[e];
   in.s[1] = a.S1[e];
   in.s[2] = a.S2[e];
@@ -352,15 +434,31 @@ __device__ __forceinline__ void stress_load(const StressArgs<T>& a, int e, StrIn
 }
 
 // S += dt*(C : grad V + m) at cell e, V gathered from V0..V2 (LDS or HBM).
-template <typename T>
+template <typename T, int KM>
 __device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrIn<T>& in, const T* V0,
                                              const T* V1, const T* V2) {
-  T sx = in.sx;
-  const T sy = in.sy, sz = in.sz;
+  // coefficients: the cell's key into the class tables, or the per-cell arrays (special
+  // cells NaN-boxed)
+  T sx, sy, sz;
+  T cc[21];
   int sp = -1;
-  if (is_boxed(sx)) {
-    sp = unbox(sx);
-    sx = a.sp_dcx[sp];
+  if (keyed<KM>(a)) {
+    const uint32_t k = resolve_key(a, in.key, sp);
+    T row[kMatRow];
+    load_row(a.cls + (size_t)(k >> kMatShift) * kMatRow, row);
+#pragma unroll
+    for (int j = 0; j < 21; j++) cc[j] = row[j];
+    axis_spacings(a.axis, k, sx, sy, sz);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 21; j++) cc[j] = in.cc[j];
+    sx = in.sx;
+    sy = in.sy;
+    sz = in.sz;
+    if (is_boxed(sx)) {
+      sp = unbox(sx);
+      sx = a.sp_dcx[sp];
+    }
   }
   // delV[i][k] = D_i V_k (i: derivative axis, k: velocity component)
   T f[8];
@@ -388,7 +486,6 @@ __device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrI
       m[k] = acc;
     }
   }
-  const T* cc = in.cc;
   T* s = in.s;
 #pragma unroll
   for (int ks = 0; ks < 6; ks++) {
@@ -414,11 +511,11 @@ __device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrI
 // Every operand that does not depend on a gather is requested first, so a cell's whole
 // streaming traffic (links, spacings, 21 C, 6 S) is in flight at once and its latency
 // overlaps the LDS staging of the chunk's unique nodes.
-template <typename T, bool LDS, int U, int CPT>
+template <typename T, bool LDS, int U, int CPT, int KM>
 __device__ __forceinline__ void stress_body(const StressArgs<T>& a, int b, unsigned char* smem) {
   const int base = b * CPT * (int)blockDim.x + (int)threadIdx.x;
   StrIn<T> in;
-  stress_load<T, LDS>(a, base < a.n ? base : a.n - 1, in);  // tail threads load a valid cell
+  stress_load<T, LDS, KM>(a, base < a.n ? base : a.n - 1, in);  // tail threads load a valid cell
   const T* V0 = a.V0;
   const T* V1 = a.V1;
   const T* V2 = a.V2;
@@ -431,27 +528,27 @@ __device__ __forceinline__ void stress_body(const StressArgs<T>& a, int b, unsig
     V2 = lds + 2 * a.max_u;
   }
   if (base >= a.n) return;
-  stress_apply(a, base, in, V0, V1, V2);
+  stress_apply<T, KM>(a, base, in, V0, V1, V2);
 #pragma unroll 1
   for (int j = 1; j < CPT; j++) {
     const int e = base + j * (int)blockDim.x;
     if (e >= a.n) break;
-    stress_load<T, LDS>(a, e, in);
-    stress_apply(a, e, in, V0, V1, V2);
+    stress_load<T, LDS, KM>(a, e, in);
+    stress_apply<T, KM>(a, e, in, V0, V1, V2);
   }
 }
 
 // Single-instance launches: one workgroup per block, XCD-contiguous block order.
-template <typename T, bool LDS, int U, int CPT>
+template <typename T, bool LDS, int U, int CPT, int KM>
 __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_kernel(VelocityArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  velocity_body<T, LDS, U, CPT>(a, a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x, smem);
+  velocity_body<T, LDS, U, CPT, KM>(a, a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x, smem);
 }
 
-template <typename T, bool LDS, int U, int CPT>
+template <typename T, bool LDS, int U, int CPT, int KM>
 __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(StressArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  stress_body<T, LDS, U, CPT>(a, a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x, smem);
+  stress_body<T, LDS, U, CPT, KM>(a, a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x, smem);
 }
 
 // Batched launches: a persistent grid walks the work table (work[0] items of (instance,
@@ -463,7 +560,7 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(S
 // of different lengths stop exactly at their own step count.  The table and the
 // descriptors live in device memory, so a captured graph of these launches stays valid
 // while instances join and leave.
-template <typename T, bool LDS, int U, int CPT>
+template <typename T, bool LDS, int U, int CPT, int KM>
 __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_batch_kernel(
     const BatchDesc<T>* __restrict__ desc, const int32_t* __restrict__ work, int k) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -477,13 +574,13 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_batc
       VelocityArgs<T> a = d->v;
       a.amp = d->amp_base + (size_t)k * d->nch;
       a.trace = d->trace_base ? d->trace_base + (size_t)(d->trace_row0 + k) * d->row : nullptr;
-      velocity_body<T, LDS, U, CPT>(a, it.y, smem);
+      velocity_body<T, LDS, U, CPT, KM>(a, it.y, smem);
     }
     __syncthreads();
   }
 }
 
-template <typename T, bool LDS, int U, int CPT>
+template <typename T, bool LDS, int U, int CPT, int KM>
 __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_batch_kernel(
     const BatchDesc<T>* __restrict__ desc, const int32_t* __restrict__ work, int k) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -496,7 +593,7 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_batch_ke
     if (k < d->kmax) {
       StressArgs<T> a = d->s;
       a.amp = d->amp_base + (size_t)k * d->nch;
-      stress_body<T, LDS, U, CPT>(a, it.y, smem);
+      stress_body<T, LDS, U, CPT, KM>(a, it.y, smem);
     }
     __syncthreads();
   }
@@ -549,16 +646,47 @@ hipError_t launch1d(K kernel, const Args& a, int n, int block, size_t lds, hipSt
 // Picks the kernel instance: direct gathers, or LDS staging with U = ceil(max_u/block)
 // entries per thread (general loop beyond 4), for 1, 2 or 4 entities per thread.
 template <typename F>
-hipError_t with_variant(bool lds, int max_u, int block, int cpt, F&& f) {
-  auto by_u = [&](auto cptc) -> hipError_t {
-    if (!lds) return f(std::false_type(), std::integral_constant<int, 0>(), cptc);
+hipError_t with_variant(bool lds, int max_u, int block, int cpt, int km, F&& f) {
+  auto by_u = [&](auto cptc, auto kmc) -> hipError_t {
+    if (!lds) return f(std::false_type(), std::integral_constant<int, 0>(), cptc, kmc);
     switch ((max_u + block - 1) / block) {
       case 0:
-      case 1: return f(std::true_type(), std::integral_constant<int, 1>(), cptc);
-      case 2: return f(std::true_type(), std::integral_constant<int, 2>(), cptc);
-      case 3: return f(std::true_type(), std::integral_constant<int, 3>(), cptc);
-      case 4: return f(std::true_type(), std::integral_constant<int, 4>(), cptc);
-      default: return f(std::true_type(), std::integral_constant<int, 0>(), cptc);
+      case 1: return f(std::true_type(), std::integral_constant<int, 1>(), cptc, kmc);
+      case 2: return f(std::true_type(), std::integral_constant<int, 2>(), cptc, kmc);
+      case 3: return f(std::true_type(), std::integral_constant<int, 3>(), cptc, kmc);
+      case 4: return f(std::true_type(), std::integral_constant<int, 4>(), cptc, kmc);
+      default: return f(std::true_type(), std::integral_constant<int, 0>(), cptc, kmc);
+    }
+  };
+  auto by_cpt = [&](auto kmc) -> hipError_t {
+    switch (cpt) {
+      case 1: return by_u(std::integral_constant<int, 1>(), kmc);
+      case 2: return by_u(std::integral_constant<int, 2>(), kmc);
+      case 4: return by_u(std::integral_constant<int, 4>(), kmc);
+      default: return hipErrorInvalidValue;
+    }
+  };
+  switch (km) {
+    case 0: return by_cpt(std::integral_constant<int, 0>());
+    case 1: return by_cpt(std::integral_constant<int, 1>());
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// The batched kernels also come in the mixed form (KM = 2).
+template <typename F>
+hipError_t with_batch_variant(bool lds, int max_u, int block, int cpt, int km, F&& f) {
+  if (km != 2) return with_variant(lds, max_u, block, cpt, km, f);
+  auto by_u = [&](auto cptc) -> hipError_t {
+    constexpr std::integral_constant<int, 2> kmc;
+    if (!lds) return f(std::false_type(), std::integral_constant<int, 0>(), cptc, kmc);
+    switch ((max_u + block - 1) / block) {
+      case 0:
+      case 1: return f(std::true_type(), std::integral_constant<int, 1>(), cptc, kmc);
+      case 2: return f(std::true_type(), std::integral_constant<int, 2>(), cptc, kmc);
+      case 3: return f(std::true_type(), std::integral_constant<int, 3>(), cptc, kmc);
+      case 4: return f(std::true_type(), std::integral_constant<int, 4>(), cptc, kmc);
+      default: return f(std::true_type(), std::integral_constant<int, 0>(), cptc, kmc);
     }
   };
   switch (cpt) {
@@ -572,18 +700,18 @@ hipError_t with_variant(bool lds, int max_u, int block, int cpt, F&& f) {
 template <typename T>
 hipError_t launch_velocity(const VelocityArgs<T>& a, int block, hipStream_t s) {
   const bool lds = a.lconn != nullptr;
-  return with_variant(lds, a.max_u, block, a.cpt, [&](auto L, auto U, auto C) {
-    return launch1d(velocity_kernel<T, decltype(L)::value, decltype(U)::value, decltype(C)::value>, a, a.n,
-                    block, lds ? (size_t)6 * a.max_u * sizeof(T) : 0, s);
+  return with_variant(lds, a.max_u, block, a.cpt, a.key ? 1 : 0, [&](auto L, auto U, auto C, auto K) {
+    return launch1d(velocity_kernel<T, decltype(L)::value, decltype(U)::value, decltype(C)::value, decltype(K)::value>,
+                    a, a.n, block, lds ? (size_t)6 * a.max_u * sizeof(T) : 0, s);
   });
 }
 
 template <typename T>
 hipError_t launch_stress(const StressArgs<T>& a, int block, hipStream_t s) {
   const bool lds = a.lconn != nullptr;
-  return with_variant(lds, a.max_u, block, a.cpt, [&](auto L, auto U, auto C) {
-    return launch1d(stress_kernel<T, decltype(L)::value, decltype(U)::value, decltype(C)::value>, a, a.n, block,
-                    lds ? (size_t)3 * a.max_u * sizeof(T) : 0, s);
+  return with_variant(lds, a.max_u, block, a.cpt, a.key ? 1 : 0, [&](auto L, auto U, auto C, auto K) {
+    return launch1d(stress_kernel<T, decltype(L)::value, decltype(U)::value, decltype(C)::value, decltype(K)::value>,
+                    a, a.n, block, lds ? (size_t)3 * a.max_u * sizeof(T) : 0, s);
   });
 }
 
@@ -603,17 +731,17 @@ hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t 
 }
 template <typename T>
 hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* work, int k, int grid, int block, bool lds,
-                        int max_u, int cpt, bool velocity, hipStream_t s, hipGraph_t graph,
+                        int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                         hipGraphNode_t* node) {
   if (grid <= 0) return hipSuccess;
   if (block < 64 || block > RSG_MAX_BLOCK || (block % 64) != 0) return hipErrorInvalidValue;
   const size_t bytes = lds ? (size_t)(velocity ? 6 : 3) * max_u * sizeof(T) : 0;
-  return with_variant(lds, max_u, block, cpt, [&](auto L, auto U, auto C) {
+  return with_batch_variant(lds, max_u, block, cpt, km, [&](auto L, auto U, auto C, auto K) {
     constexpr bool l = decltype(L)::value;
-    constexpr int u = decltype(U)::value, c = decltype(C)::value;
+    constexpr int u = decltype(U)::value, c = decltype(C)::value, m = decltype(K)::value;
     if (graph) {
-      const void* fn = velocity ? reinterpret_cast<const void*>(&velocity_batch_kernel<T, l, u, c>)
-                                : reinterpret_cast<const void*>(&stress_batch_kernel<T, l, u, c>);
+      const void* fn = velocity ? reinterpret_cast<const void*>(&velocity_batch_kernel<T, l, u, c, m>)
+                                : reinterpret_cast<const void*>(&stress_batch_kernel<T, l, u, c, m>);
       void* args[3] = {(void*)&desc, (void*)&work, (void*)&k};
       hipKernelNodeParams p{};
       p.func = const_cast<void*>(fn);
@@ -626,22 +754,22 @@ hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* work, int k, in
       return hipGraphAddKernelNode(node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &p);
     }
     if (velocity)
-      hipLaunchKernelGGL((velocity_batch_kernel<T, l, u, c>), dim3(grid), dim3(block), bytes, s, desc, work, k);
+      hipLaunchKernelGGL((velocity_batch_kernel<T, l, u, c, m>), dim3(grid), dim3(block), bytes, s, desc, work, k);
     else
-      hipLaunchKernelGGL((stress_batch_kernel<T, l, u, c>), dim3(grid), dim3(block), bytes, s, desc, work, k);
+      hipLaunchKernelGGL((stress_batch_kernel<T, l, u, c, m>), dim3(grid), dim3(block), bytes, s, desc, work, k);
     return hipGetLastError();
   });
 }
 
 hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* work, int k, int grid, int block,
-                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s, hipGraph_t graph,
+                            bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node) {
-  return launch_batch<double>(d, work, k, grid, block, lds, max_u, cpt, velocity, s, graph, node);
+  return launch_batch<double>(d, work, k, grid, block, lds, max_u, cpt, km, velocity, s, graph, node);
 }
 hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* work, int k, int grid, int block,
-                            bool lds, int max_u, int cpt, bool velocity, hipStream_t s, hipGraph_t graph,
+                            bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node) {
-  return launch_batch<float>(d, work, k, grid, block, lds, max_u, cpt, velocity, s, graph, node);
+  return launch_batch<float>(d, work, k, grid, block, lds, max_u, cpt, km, velocity, s, graph, node);
 }
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(energy_kernel<double>, dim3(nblocks), dim3(256), 0, s, a);

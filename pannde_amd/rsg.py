@@ -65,7 +65,8 @@ class TuningDesc(ctypes.Structure):
                 ("per_thread_velocity", ctypes.c_int32), ("per_thread_stress", ctypes.c_int32),
                 ("lds_velocity", ctypes.c_int32), ("lds_stress", ctypes.c_int32), ("tile", ctypes.c_int32 * 3),
                 ("c_layout", ctypes.c_int32),
-                ("xcd_remap", ctypes.c_int32)]
+                ("xcd_remap", ctypes.c_int32), ("coeff_classes", ctypes.c_int32),
+                ("node_classes", ctypes.c_int32), ("cell_classes", ctypes.c_int32)]
 
 
 def tuning_desc(tuning):
@@ -239,7 +240,8 @@ class DeviceSolver:
         return dict(block_velocity=t.block_velocity, block_stress=t.block_stress,
                     per_thread_velocity=t.per_thread_velocity, per_thread_stress=t.per_thread_stress,
                     lds_velocity=t.lds_velocity, lds_stress=t.lds_stress, tile=tuple(t.tile),
-                    c_layout=t.c_layout, xcd_remap=t.xcd_remap)
+                    c_layout=t.c_layout, xcd_remap=t.xcd_remap, coeff_classes=t.coeff_classes,
+                    node_classes=t.node_classes, cell_classes=t.cell_classes)
 
     # -- sources / sensors
     def set_sources(self, force=(), moment=()):

@@ -42,7 +42,9 @@ def test_batch_equals_alone(precision, use_graph):
     jobs = mix()
     solo = [alone(c, n, precision) for c, n in jobs]
     b = rsg.Batch(precision, steps_per_block=16, use_graph=use_graph)
-    solvers = [rsg.DeviceSolver.from_case(c, precision) for c, _ in jobs]
+    # every other member keeps per-entity coefficient arrays: the two forms share launches
+    solvers = [rsg.DeviceSolver.from_case(c, precision, tuning=dict(coeff_classes=-1) if i % 2 else None)
+               for i, (c, _) in enumerate(jobs)]
     slots = [b.add(s, n) for s, (_, n) in zip(solvers, jobs)]
     left = b.run(1000)
     b.synchronize()

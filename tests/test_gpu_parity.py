@@ -69,6 +69,8 @@ TUNINGS = [
     dict(lds_velocity=1, lds_stress=-1, per_thread_velocity=2, per_thread_stress=4),
     dict(tile=(1 << 20, 1, 1)),
     dict(tile=(16, 4, 4), c_layout=-1, xcd_remap=-1),
+    dict(coeff_classes=-1),
+    dict(coeff_classes=-1, lds_velocity=-1, per_thread_stress=2, c_layout=-1),
 ]
 
 
@@ -83,6 +85,33 @@ def test_tuning_does_not_change_results(name, tuning):
         assert_same((V, S, t, s.traces()), (ref["V"], ref["S"], ref["time"], ref["traces"]))
     finally:
         s.close()
+
+
+def test_coefficient_class_tables_in_use():
+    """Plies and homogeneous blocks share material rows; per-cell random C falls back to
+    per-cell arrays; the fallback can also be asked for."""
+    plate, _ = load_golden("cfrp_holes_rcm")  # 16 plies in 4 orientations, [0/45/90/-45]2s
+    s = rsg.DeviceSolver.from_case(plate)
+    t = s.tuning()
+    s.close()
+    # node materials: dt/rho at interior, face, edge, corner and hole-rim nodes (the /8 rule)
+    assert t["coeff_classes"] == 1 and t["cell_classes"] == 4 and 0 < t["node_classes"] <= 8
+    s = rsg.DeviceSolver.from_case(plate, tuning=dict(coeff_classes=-1))
+    t = s.tuning()
+    s.close()
+    assert t["coeff_classes"] == -1 and t["cell_classes"] == -1 and t["node_classes"] == -1
+    rng = np.random.default_rng(5)
+    case = cases.al_block(seed=5, nx=30, ny=30, nz=20, nsteps=20, n_sensors=4)
+    for f in case.cij:
+        f.data = f.data * (1 + 0.05 * rng.standard_normal(f.data.shape))
+    s = rsg.DeviceSolver.from_case(case)
+    t = s.tuning()
+    s.solve(case.nsteps)
+    dev = s.get_state()
+    s.close()
+    assert t["cell_classes"] == -1 and t["node_classes"] > 0  # 18000 distinct rows > 18000/64
+    ref = orc.run_oracle(case)
+    assert np.array_equal(dev[0], ref[0]) and np.array_equal(dev[1], ref[1])
 
 
 @pytest.mark.parametrize("tuning", [dict(block_stress=1024), dict(block_velocity=96), dict(per_thread_stress=3)])
