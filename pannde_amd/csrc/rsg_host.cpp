@@ -17,6 +17,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -24,8 +25,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <unordered_map>
 #include <vector>
@@ -46,6 +49,25 @@ int fail(int code, const char* fmt, ...) {
   g_last_error = buf;
   return code;
 }
+
+// Set-up phase timing on stderr, compiled in only with -DRSG_CREATE_TIMING.
+#ifdef RSG_CREATE_TIMING
+inline std::chrono::steady_clock::time_point& create_clock() {
+  static thread_local std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  return t;
+}
+#define CREATE_PHASE(name)                                                                        \
+  do {                                                                                           \
+    const auto t_now_ = std::chrono::steady_clock::now();                                        \
+    fprintf(stderr, "rsg_create %-12s %8.1f ms\n", name,                                         \
+            std::chrono::duration<double, std::milli>(t_now_ - create_clock()).count());         \
+    create_clock() = t_now_;                                                                     \
+  } while (0)
+#else
+#define CREATE_PHASE(name) \
+  do {                     \
+  } while (0)
+#endif
 
 #define HIP_TRY(expr)                                                                    \
   do {                                                                                   \
@@ -213,6 +235,8 @@ struct rsg_instance {
 
   size_t elt() const { return precision == RSG_FP64 ? 8 : 4; }
 
+  std::mutex alloc_mu;  // set-up allocates from two threads
+
   int dalloc(void** p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) bytes = 8;
@@ -221,6 +245,7 @@ struct rsg_instance {
       *p = nullptr;
       return fail(RSG_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
     }
+    std::lock_guard<std::mutex> lock(alloc_mu);
     allocations.push_back(*p);
     dev_bytes += (int64_t)bytes;
     return RSG_OK;
@@ -307,6 +332,44 @@ std::vector<int32_t> locality_order(int32_t n, const std::vector<double>& px,
                          std::min<uint64_t>((uint64_t)tile[1], ext[1]),
                          std::min<uint64_t>((uint64_t)tile[2], ext[2])};
   const uint64_t nb[3] = {(ext[0] + t[0] - 1) / t[0], (ext[1] + t[1] - 1) / t[1], (ext[2] + t[2] - 1) / t[2]};
+  std::vector<int32_t> order((size_t)n);
+  const long double span = (long double)(nb[0] * t[0]) * (long double)(nb[1] * t[1]) * (long double)(nb[2] * t[2]);
+  if (span < 9.2e18L) {
+    // one key per point, brick * (brick volume) + inner < span: a stable LSD radix sort on
+    // the bits the largest key uses keeps equal keys in index order
+    const uint64_t vol = t[0] * t[1] * t[2];
+    std::vector<uint64_t> k((size_t)n), k2((size_t)n);
+    std::vector<int32_t> o2((size_t)n);
+    uint64_t kmax = 0;
+    for (int32_t i = 0; i < n; i++) {
+      const uint64_t qx = q[(size_t)i * 3], qy = q[(size_t)i * 3 + 1], qz = q[(size_t)i * 3 + 2];
+      const uint64_t brick = ((qz / t[2]) * nb[1] + qy / t[1]) * nb[0] + qx / t[0];
+      const uint64_t inner = ((qz % t[2]) * t[1] + qy % t[1]) * t[0] + qx % t[0];
+      k[i] = brick * vol + inner;
+      kmax = std::max(kmax, k[i]);
+      order[i] = i;
+    }
+    constexpr int kDigit = 11;
+    std::vector<int32_t> count((size_t)1 << kDigit);
+    for (int shift = 0; shift < 64 && (kmax >> shift) != 0; shift += kDigit) {
+      std::fill(count.begin(), count.end(), 0);
+      for (int32_t i = 0; i < n; i++) count[(k[i] >> shift) & ((1u << kDigit) - 1)]++;
+      int32_t sum = 0;
+      for (auto& c : count) {
+        const int32_t v = c;
+        c = sum;
+        sum += v;
+      }
+      for (int32_t i = 0; i < n; i++) {
+        const int32_t dst = count[(k[i] >> shift) & ((1u << kDigit) - 1)]++;
+        k2[dst] = k[i];
+        o2[dst] = order[i];
+      }
+      k.swap(k2);
+      order.swap(o2);
+    }
+    return order;
+  }
   std::vector<std::pair<std::pair<uint64_t, uint64_t>, int32_t>> key((size_t)n);
   for (int32_t i = 0; i < n; i++) {
     const uint64_t qx = q[(size_t)i * 3], qy = q[(size_t)i * 3 + 1], qz = q[(size_t)i * 3 + 2];
@@ -315,7 +378,6 @@ std::vector<int32_t> locality_order(int32_t n, const std::vector<double>& px,
     key[i] = {{brick, inner}, i};
   }
   std::sort(key.begin(), key.end());
-  std::vector<int32_t> order((size_t)n);
   for (int32_t i = 0; i < n; i++) order[i] = key[i].second;
   return order;
 }
@@ -330,25 +392,31 @@ struct LdsPlan {
   int32_t max_u = 0;
 };
 
-bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int block, LdsPlan& p) {
+bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other, int block, LdsPlan& p) {
   const int32_t nb = (n + block - 1) / block;
   p.u_off.assign((size_t)nb + 1, 0);
   p.ulist.clear();
+  p.ulist.reserve((size_t)n * 2);
   p.lconn.assign((size_t)n * 8, 0xFFFF);
   p.max_u = 0;
+  std::vector<int32_t> seen((size_t)n_other, -1), local((size_t)n_other);
   std::vector<int32_t> ids;
   ids.reserve((size_t)block * 8);
   for (int32_t b = 0; b < nb; b++) {
     const int32_t lo = b * block, hi = std::min(n, lo + block);
     ids.clear();
-    for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++)
-      if (conn[i] >= 0) ids.push_back(conn[i]);
+    for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++) {
+      const int32_t g = conn[i];
+      if (g >= 0 && seen[g] != b) {
+        seen[g] = b;
+        ids.push_back(g);
+      }
+    }
     std::sort(ids.begin(), ids.end());
-    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
     if (ids.size() >= 0xFFFF) return false;
+    for (size_t k = 0; k < ids.size(); k++) local[ids[k]] = (int32_t)k;
     for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++)
-      if (conn[i] >= 0)
-        p.lconn[i] = (uint16_t)(std::lower_bound(ids.begin(), ids.end(), conn[i]) - ids.begin());
+      if (conn[i] >= 0) p.lconn[i] = (uint16_t)local[conn[i]];
     p.ulist.insert(p.ulist.end(), ids.begin(), ids.end());
     p.u_off[b + 1] = (int32_t)p.ulist.size();
     p.max_u = std::max(p.max_u, (int32_t)ids.size());
@@ -838,6 +906,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   for (int c : {h->cpt_v, h->cpt_s})
     if (c != 1 && c != 2 && c != 4) return fail(RSG_ERR_ARG, "entities per thread %d: must be 1, 2 or 4", c);
 
+  CREATE_PHASE("start");
   // links (HexMesh::buildLinks)
   h->cells.assign(mesh->cell_nodes, mesh->cell_nodes + (size_t)nc * 8);
   h->node_cells.assign((size_t)nn * 8, -1);
@@ -845,6 +914,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     for (int v = 0; v < 8; v++) h->node_cells[(size_t)h->cells[(size_t)k * 8 + v] * 8 + (v ^ 6)] = k;
 
   const double* X[3] = {mesh->x, mesh->y, mesh->z};
+  CREATE_PHASE("links");
   // RightHex::cellPrep
   std::vector<double> dcell[3];
   for (int d = 0; d < 3; d++) dcell[d].resize(nc);
@@ -852,6 +922,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     const int32_t a = h->cells[(size_t)c * 8], b = h->cells[(size_t)c * 8 + 6];
     for (int d = 0; d < 3; d++) dcell[d][c] = X[d][b] - X[d][a];
   }
+  CREATE_PHASE("cellprep");
   // RightHex::nodePrep
   std::vector<double> dnode[3];
   for (int d = 0; d < 3; d++) dnode[d].resize(nn);
@@ -881,6 +952,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     }
   }
 
+  CREATE_PHASE("nodeprep");
   // locality order
   double lo[3], hmin[3];
   for (int d = 0; d < 3; d++) {
@@ -912,6 +984,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   for (int32_t i = 0; i < nc; i++) h->cell_at[h->cell_of[i]] = i;
   for (int32_t i = 0; i < nn; i++) h->node_at[h->node_of[i]] = i;
 
+  CREATE_PHASE("order");
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   size_t lds_limit = 0;
   {
@@ -921,8 +994,10 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     lds_limit = (size_t)std::min(bytes, 64 * 1024);
   }
 
-  // node arrays
-  {
+  CREATE_PHASE("stream");
+  // node arrays and cell arrays are independent: built on two threads
+  auto node_part = [&]() -> int {
+    int rc;
     std::vector<double> idt(nn), w(nn), dn[3];
     for (int d = 0; d < 3; d++) dn[d].resize(nn);
     std::vector<int32_t> conn((size_t)nn * 8);
@@ -963,7 +1038,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->ncell = (int32_t*)p;
     LdsPlan plan;
-    if (h->lds_v && build_lds_plan(conn, nn, h->block_v * h->cpt_v, plan) &&
+    if (h->lds_v && build_lds_plan(conn, nn, nc, h->block_v * h->cpt_v, plan) &&
         (size_t)6 * plan.max_u * h->elt() <= lds_limit) {
       if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
       h->v_uoff = (int32_t*)p;
@@ -975,9 +1050,10 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     }
     if ((rc = upload(h.get(), &p, w))) return rc;
     h->node_w = (double*)p;
-  }
-  // cell arrays
-  {
+    return RSG_OK;
+  };
+  auto cell_part = [&]() -> int {
+    int rc;
     std::vector<double> dc[3], w(nc);
     for (int d = 0; d < 3; d++) dc[d].resize(nc);
     std::vector<int32_t> conn((size_t)nc * 8);
@@ -1024,7 +1100,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->cnode = (int32_t*)p;
     LdsPlan plan;
-    if (h->lds_s && build_lds_plan(conn, nc, h->block_s * h->cpt_s, plan) &&
+    if (h->lds_s && build_lds_plan(conn, nc, nn, h->block_s * h->cpt_s, plan) &&
         (size_t)3 * plan.max_u * h->elt() <= lds_limit) {
       if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
       h->s_uoff = (int32_t*)p;
@@ -1036,7 +1112,37 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     }
     if ((rc = upload(h.get(), &p, w))) return rc;
     h->cell_w = (double*)p;
+    return RSG_OK;
+  };
+  {
+    int rc_cells = RSG_OK;
+    std::string err_cells;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::thread worker([&]() {
+      try {
+        // a new thread starts on device 0: work on the caller's device
+        const hipError_t e = hipSetDevice(dev);
+        rc_cells = e != hipSuccess ? fail(RSG_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e)) : cell_part();
+      } catch (const std::bad_alloc&) {
+        rc_cells = fail(RSG_ERR_NOMEM, "host allocation failed");
+      }
+      if (rc_cells) err_cells = g_last_error;
+    });
+    int rc_nodes;
+    try {
+      rc_nodes = node_part();
+    } catch (const std::bad_alloc&) {
+      rc_nodes = fail(RSG_ERR_NOMEM, "host allocation failed");
+    }
+    worker.join();
+    if (rc_nodes) return rc_nodes;
+    if (rc_cells) {
+      g_last_error = err_cells;
+      return rc_cells;
+    }
   }
+  CREATE_PHASE("arrays");
   // zero state (TransducerModel.cpp:150-159)
   for (int d = 0; d < 3; d++) {
     if ((rc = h->dalloc(&h->V[d], (size_t)nn * h->elt()))) return rc;
@@ -1048,6 +1154,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   }
   if ((rc = h->partial.alloc(2 * 1024))) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));  // this handle's work only (others may be running)
+  CREATE_PHASE("state");
   *out = h.release();
   return RSG_OK;
 }
