@@ -126,7 +126,7 @@ def served(rsg, n, rate, seed):
     srv = serve.Server(log=log)
     rep = srv.serve(reqs)
     keys = ("requests", "elapsed_s", "solves_per_hour", "latency_p50_s", "latency_p99_s", "latency_mean_s",
-            "gcell_steps_per_s", "gcell_steps_per_s_with_checks", "gpu_busy_fraction", "prep_seconds_mean",
+            "gcell_steps_per_s", "gcell_steps_per_s_with_checks", "blocks", "prep_seconds_mean",
             "spot_checks", "spot_check_max_trace_rel_l2")
     out = {k: (float("%.4g" % rep[k]) if isinstance(rep[k], float) else rep[k]) for k in keys}
     out.update(arrival_rate_per_s=rate, seed=seed, hbm_budget_gib=srv.hbm_budget / (1 << 30),

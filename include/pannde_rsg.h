@@ -223,10 +223,13 @@ typedef struct rsg_batch_s* rsg_batch;
 int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, int32_t use_graph,
                      rsg_batch* out);
 
-/* Adds a handle that will take nsteps more steps; *slot receives its slot id. */
+/* Adds a handle that will take nsteps more steps; *slot receives its slot id.  It joins at
+ * the next K-step block; blocks already enqueued keep running. */
 int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot);
 
-/* Detaches a slot's handle (finished or not); the handle can be stepped alone again. */
+/* Detaches a slot's handle (finished or not); the handle can be stepped alone again.
+ * A finished instance waits only for the block in which it finished, so later blocks of
+ * the batch can run meanwhile; an unfinished one waits for the batch's stream. */
 int rsg_batch_remove(rsg_batch b, int32_t slot);
 
 /* Runs up to nblocks K-step blocks (enqueued; see rsg_batch_synchronize).  *unfinished

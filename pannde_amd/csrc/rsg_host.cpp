@@ -1497,6 +1497,8 @@ struct rsg_batch_s {
     rsg_instance* h = nullptr;
     int64_t total = 0, done = 0;
     bool attached = false;
+    hipEvent_t fin = nullptr;   // recorded after the block in which the instance finishes
+    bool fin_recorded = false;
   };
   std::vector<Slot> slots;
   // kernel configuration shared by every instance
@@ -1541,11 +1543,13 @@ struct rsg_batch_s {
   }
   ~rsg_batch_s() {
     if (stream) (void)hipStreamSynchronize(stream);
-    for (auto& sl : slots)
+    for (auto& sl : slots) {
       if (sl.attached && sl.h) {
         sl.h->in_batch = false;
         sl.h->batch_stream = nullptr;
       }
+      if (sl.fin) (void)hipEventDestroy(sl.fin);
+    }
     drop_graph();
     if (ctl_dev) (void)hipFree(ctl_dev);
     for (int i = 0; i < 2; i++) {
@@ -1795,7 +1799,12 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
     h->steps += kmax[i];
     if (!h->sensors.empty()) h->trace_dev_steps += kmax[i];
     sl.done += kmax[i];
-    if (sl.done < sl.total) left++;
+    if (sl.done < sl.total) {
+      left++;
+    } else if (!sl.fin_recorded) {  // later blocks leave this instance alone
+      HIP_TRY(hipEventRecord(sl.fin, b->stream));
+      sl.fin_recorded = true;
+    }
   }
   *unfinished = left;
   return RSG_OK;
@@ -1849,8 +1858,8 @@ int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot) {
   }
   int rc;
   if (h->tables_dirty && (rc = build_tables(h))) return rc;
+  // the handle's own uploads must be complete; blocks already enqueued do not involve it
   HIP_TRY(hipStreamSynchronize(h->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
   // one kernel variant serves the batch: LDS staging only if every instance has a plan,
   // sized for the largest staged set
   const bool lv = b->lds_v && h->v_lconn, ls = b->lds_s && h->s_lconn;
@@ -1871,10 +1880,12 @@ int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot) {
     b->slots.emplace_back();
   }
   auto& sl = b->slots[id];
+  if (!sl.fin) HIP_TRY(hipEventCreateWithFlags(&sl.fin, hipEventDisableTiming));
   sl.h = h;
   sl.total = nsteps;
   sl.done = 0;
   sl.attached = true;
+  sl.fin_recorded = false;
   h->in_batch = true;
   h->batch_stream = b->stream;
   *slot = id;
@@ -1884,11 +1895,18 @@ int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot) {
 int rsg_batch_remove(rsg_batch b, int32_t slot) {
   if (!b || slot < 0 || slot >= (int32_t)b->slots.size() || !b->slots[slot].attached)
     return fail(RSG_ERR_ARG, "no such slot %d", slot);
-  HIP_TRY(hipStreamSynchronize(b->stream));
   auto& sl = b->slots[slot];
+  // a finished instance waits only for the block in which it finished
+  if (sl.fin_recorded)
+    HIP_TRY(hipEventSynchronize(sl.fin));
+  else
+    HIP_TRY(hipStreamSynchronize(b->stream));
   sl.h->in_batch = false;
   sl.h->batch_stream = nullptr;
-  sl = rsg_batch_s::Slot();
+  sl.h = nullptr;
+  sl.total = sl.done = 0;
+  sl.attached = false;
+  sl.fin_recorded = false;
   return RSG_OK;
 }
 

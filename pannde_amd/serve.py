@@ -150,44 +150,44 @@ class Server:
         return case, s, time.perf_counter() - t0
 
     def serve(self, requests: List[Request], time_scale: float = 1.0) -> Dict:
+        """Serves the requests (arrival times scaled by ``time_scale``) and returns the report.
+
+        The loop keeps the GPU fed: it enqueues the next K-step block of every batch before it
+        retires the solves that finished in the previous block (whose results only need that
+        block), admits and folds in newcomers, and hands solver tear-down to the pool."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         pool = cf.ThreadPoolExecutor(max_workers=self.prep_threads)
         main = rsg.Batch(self.precision, steps_per_block=self.K)
         check = rsg.Batch(rsg.RSG_FP64, steps_per_block=self.K)
         start = time.perf_counter()
         now = lambda: time.perf_counter() - start  # noqa: E731
-        nxt = 0  # next request not yet admitted
-        used_bytes = 0
-        resident_cells = 0
+        st = dict(nxt=0, used_bytes=0, resident_cells=0, cell_steps=0, blocks=0)
         preps = {}  # future -> (request, bytes, admitted time, is spot check)
-        active: Dict[int, _Job] = {}
-        checking: Dict[int, _Job] = {}
+        active: Dict[int, _Job] = {}  # main batch: slot -> job
+        checking: Dict[int, _Job] = {}  # FP64 batch: slot -> job
         results: Dict[int, Dict] = {}
         spot_errors = []
-        gpu_busy = 0.0
         prep_seconds = []
-        cell_steps = 0
-        last_log = 0.0
+        last_log = -1e9
 
         def admit():
-            nonlocal nxt, used_bytes, resident_cells
-            while nxt < len(reqs) and reqs[nxt].arrival * time_scale <= now():
-                r = reqs[nxt]
+            while st["nxt"] < len(reqs) and reqs[st["nxt"]].arrival * time_scale <= now():
+                r = reqs[st["nxt"]]
                 nc = 2 * r.cells  # upper bound before the mesh exists (holes, rounding)
                 est = estimate_device_bytes(nc, int(1.1 * nc), self.precision)
-                busy = used_bytes > 0 or resident_cells > 0
-                if busy and (used_bytes + est > self.hbm_budget or resident_cells + r.cells > self.max_resident_cells):
+                busy = st["used_bytes"] > 0 or st["resident_cells"] > 0
+                if busy and (st["used_bytes"] + est > self.hbm_budget
+                             or st["resident_cells"] + r.cells > self.max_resident_cells):
                     return
-                used_bytes += est
-                resident_cells += r.cells
+                st["used_bytes"] += est
+                st["resident_cells"] += r.cells
                 preps[pool.submit(self._prepare, r, self.precision)] = (r, est, now(), False)
-                nxt += 1
+                st["nxt"] += 1
 
-        def collect(block: bool, timeout: float):
-            nonlocal used_bytes
+        def collect(timeout):
             if not preps:
                 return
-            done, _ = cf.wait(list(preps), timeout=timeout if block else 0, return_when=cf.FIRST_COMPLETED)
+            done, _ = cf.wait(list(preps), timeout=timeout, return_when=cf.FIRST_COMPLETED)
             for f in done:
                 r, est, t_adm, is_check = preps.pop(f)
                 case, s, secs = f.result()
@@ -201,75 +201,81 @@ class Server:
                     job.started = now()
                     active[job.slot] = job
 
-        while nxt < len(reqs) or preps or active or checking:
+        def enqueue(batch, jobs):
+            """One K-step block if any member has steps left; returns the slots that finish in it."""
+            if not any(batch.status(k)[0] < batch.status(k)[1] for k in jobs):
+                return []
+            before = {k for k in jobs if batch.status(k)[0] >= batch.status(k)[1]}
+            batch.run(1)
+            st["blocks"] += 1
+            return [k for k in jobs if k not in before and batch.status(k)[0] >= batch.status(k)[1]]
+
+        def retire_main(slot):
+            job = active.pop(slot)
+            main.remove(slot)  # waits only for the block in which it finished
+            tr = job.solver.traces()
+            t_done = now()
+            r = job.req
+            st["cell_steps"] += job.case.n_cells * r.nsteps
+            st["resident_cells"] -= r.cells
+            results[r.rid] = dict(rid=r.rid, kind=r.kind, cells=job.case.n_cells, nsteps=r.nsteps,
+                                  arrival=r.arrival * time_scale, admitted=job.admitted, ready=job.ready,
+                                  finished=t_done, latency=t_done - r.arrival * time_scale)
+            if self.keep_traces:
+                results[r.rid]["traces"] = tr
+            pool.submit(job.solver.close)
+            if r.spot_check:  # the FP64 re-run shares the byte budget; its solve reuses the case
+                est = estimate_device_bytes(job.case.n_cells, job.case.n_nodes, rsg.RSG_FP64)
+                st["used_bytes"] += est - job.bytes
+                preps[pool.submit(self._prepare, r, rsg.RSG_FP64, job.case)] = (r, est, t_done, True)
+                results[r.rid]["fp32_traces"] = tr
+            else:
+                st["used_bytes"] -= job.bytes
+
+        def retire_check(slot):
+            job = checking.pop(slot)
+            check.remove(slot)
+            tr64 = job.solver.traces()
+            res = results[job.req.rid]
+            err = rel_l2_traces(res.pop("fp32_traces"), tr64)
+            # None when no sensor saw the wave in the FP64 run (nothing to compare)
+            res["spot_check_trace_rel_l2"] = float(err.max()) if err.size else None
+            res["spot_check_sensors"] = int(err.size)
+            if err.size:
+                spot_errors.append(res["spot_check_trace_rel_l2"])
+            st["cell_steps"] += job.case.n_cells * job.req.nsteps
+            st["used_bytes"] -= job.bytes
+            pool.submit(job.solver.close)
+
+        pending_main, pending_check = [], []
+        while st["nxt"] < len(reqs) or preps or active or checking:
             if now() - last_log >= 15.0:
                 last_log = now()
                 self.log("serve: t=%.0f s, %d done, %d active, %d preparing, %d checking, %d waiting"
                          % (last_log, len(results) - len(checking), len(active), len(preps), len(checking),
-                            len(reqs) - nxt))
+                            len(reqs) - st["nxt"]))
             admit()
-            collect(block=False, timeout=0)
-            if not active and not checking:
-                wait = None
-                if nxt < len(reqs):
-                    wait = max(0.0, reqs[nxt].arrival * time_scale - now())
-                if preps:
-                    collect(block=True, timeout=wait if wait is not None else 1.0)
-                elif wait is not None:
-                    time.sleep(min(wait, 0.05))
-                continue
-            t0 = time.perf_counter()
-            if active:
-                main.run(1)
-            if checking:
-                check.run(1)
-            main.synchronize()
-            check.synchronize()
-            gpu_busy += time.perf_counter() - t0
-            t_done = now()
-            for slot in list(active):
-                job = active[slot]
-                done, total = main.status(slot)
-                if done < total:
-                    continue
-                tr = job.solver.traces()
-                main.remove(slot)
-                del active[slot]
-                r = job.req
-                cell_steps += job.case.n_cells * r.nsteps
-                resident_cells -= r.cells
-                results[r.rid] = dict(rid=r.rid, kind=r.kind, cells=job.case.n_cells, nsteps=r.nsteps,
-                                      arrival=r.arrival * time_scale, admitted=job.admitted, ready=job.ready,
-                                      finished=t_done, latency=t_done - r.arrival * time_scale)
-                if self.keep_traces:
-                    results[r.rid]["traces"] = tr
-                job.solver.close()
-                if r.spot_check:  # the FP64 re-run shares the byte budget; its solve reuses the case
-                    est = estimate_device_bytes(job.case.n_cells, job.case.n_nodes, rsg.RSG_FP64)
-                    used_bytes += est - job.bytes
-                    f = pool.submit(self._prepare, r, rsg.RSG_FP64, job.case)
-                    preps[f] = (r, est, t_done, True)
-                    results[r.rid]["fp32_traces"] = tr
-                else:
-                    used_bytes -= job.bytes
-            for slot in list(checking):
-                job = checking[slot]
-                done, total = check.status(slot)
-                if done < total:
-                    continue
-                tr64 = job.solver.traces()
-                check.remove(slot)
-                del checking[slot]
-                res = results[job.req.rid]
-                err = rel_l2_traces(res.pop("fp32_traces"), tr64)
-                # None when no sensor saw the wave in the FP64 run (nothing to compare)
-                res["spot_check_trace_rel_l2"] = float(err.max()) if err.size else None
-                res["spot_check_sensors"] = int(err.size)
-                if err.size:
-                    spot_errors.append(res["spot_check_trace_rel_l2"])
-                cell_steps += job.case.n_cells * job.req.nsteps
-                used_bytes -= job.bytes
-                job.solver.close()
+            collect(0)
+            new_main = enqueue(main, active)
+            new_check = enqueue(check, checking)
+            for slot in pending_main:
+                retire_main(slot)
+            for slot in pending_check:
+                retire_check(slot)
+            pending_main, pending_check = new_main, new_check
+            if not new_main and not new_check and not pending_main and not pending_check:
+                if not any(main.status(k)[0] < main.status(k)[1] for k in active) and \
+                        not any(check.status(k)[0] < check.status(k)[1] for k in checking):
+                    # nothing on the GPU: wait for a solver to be ready or the next arrival
+                    wait = None
+                    if st["nxt"] < len(reqs):
+                        wait = max(0.0, reqs[st["nxt"]].arrival * time_scale - now())
+                    if preps:
+                        collect(wait if wait is not None else 1.0)
+                    elif wait is not None:
+                        time.sleep(min(wait, 0.05))
+        main.synchronize()
+        check.synchronize()
         elapsed = now()
         main.close()
         check.close()
@@ -281,8 +287,8 @@ class Server:
             latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),
             latency_mean_s=float(lat.mean()),
             gcell_steps_per_s=served_cell_steps / elapsed / 1e9,
-            gcell_steps_per_s_with_checks=cell_steps / elapsed / 1e9,
-            gpu_busy_fraction=gpu_busy / elapsed,
+            gcell_steps_per_s_with_checks=st["cell_steps"] / elapsed / 1e9,
+            blocks=st["blocks"],
             prep_seconds_mean=float(np.mean(prep_seconds)) if prep_seconds else 0.0,
             spot_checks=sum(1 for r in reqs if r.spot_check),
             spot_check_max_trace_rel_l2=max(spot_errors) if spot_errors else None,
