@@ -1,0 +1,180 @@
+"""VTK XML reader/writer (no VTK in this image): round trips over every encoding, header
+type and compression, plus files written by hand from the format's definition (ascii,
+big-endian inline base64, a zlib block header built field by field, a voxel cell), and the
+job conversion (fields, transducers, timing, sensors) against the oracle's schedule."""
+import base64
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle_rsg as orc
+from pannde_amd import cases, vtkio
+
+
+def two_hexes():
+    x, y, z, cells, _ = cases.structured_mesh(2, 1, 1, 0.5, 1.0, 2.0)
+    return np.stack([x, y, z], 1), cells
+
+
+@pytest.mark.parametrize("fmt", ["ascii", "binary", "appended"])
+@pytest.mark.parametrize("header", ["UInt32", "UInt64"])
+@pytest.mark.parametrize("compress", [False, True])
+def test_round_trip(tmp_path, fmt, header, compress):
+    if fmt == "ascii" and compress:
+        pytest.skip("ascii data is never compressed")
+    rng = np.random.default_rng(3)
+    pts, cells = two_hexes()
+    pdata = {"density": rng.standard_normal(pts.shape[0]), "f32": rng.standard_normal(pts.shape[0]).astype(np.float32),
+             "vec": rng.standard_normal((pts.shape[0], 3)), "ids": np.arange(pts.shape[0], dtype=np.int32) - 5}
+    cdata = {"C11": rng.standard_normal(2) * 1e11, "u8": np.array([7, 250], np.uint8), "i64": np.array([-2 ** 40, 3])}
+    fdata = {"dt": np.array([1.2345e-9]), "write_times": np.linspace(0, 1e-6, 7), "XD0/Radius": np.array([2e-3])}
+    p = tmp_path / "a.vtu"
+    vtkio.write_vtu(str(p), pts, cells, pdata, cdata, fdata, fmt=fmt, header=header, compress=compress)
+    v = vtkio.read_vtu(str(p))
+    assert np.array_equal(v.points, pts) and np.array_equal(v.cells, cells)
+    for src, dst in ((pdata, v.point_data), (cdata, v.cell_data), (fdata, v.field_data)):
+        for k, a in src.items():
+            assert dst[k].dtype == np.asarray(a).dtype, k
+            assert np.array_equal(dst[k], a), k
+    assert v.value("dt") == 1.2345e-9
+    with pytest.raises(vtkio.VtkError, match="scalar"):
+        v.value("write_times")
+
+
+def test_large_compressed_arrays_use_several_blocks(tmp_path):
+    pts, cells = two_hexes()
+    big = np.random.default_rng(1).standard_normal(20000)  # 160 kB: five 32 kB blocks
+    p = tmp_path / "b.vtu"
+    vtkio.write_vtu(str(p), pts, cells, field_data={"big": big}, fmt="appended", compress=True)
+    assert np.array_equal(vtkio.read_vtu(str(p)).field_data["big"], big)
+
+
+ASCII_JOB = """<?xml version="1.0"?>
+<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian">
+  <UnstructuredGrid>
+    <FieldData>
+      <DataArray type="Float64" Name="dt" NumberOfTuples="1" format="ascii">2.5e-08</DataArray>
+    </FieldData>
+    <Piece NumberOfPoints="12" NumberOfCells="2">
+      <PointData>
+        <DataArray type="Float32" Name="density" format="ascii">1 2 3 4 5 6 7 8 9 10 11 12</DataArray>
+      </PointData>
+      <CellData>
+        <DataArray type="Float64" Name="density" format="ascii">100 200</DataArray>
+        <DataArray type="Float64" Name="C11" format="ascii">1.5e11 2.5e11</DataArray>
+      </CellData>
+      <Points>
+        <DataArray type="Float64" NumberOfComponents="3" format="ascii">
+          0 0 0  1 0 0  2 0 0  0 1 0  1 1 0  2 1 0
+          0 0 1  1 0 1  2 0 1  0 1 1  1 1 1  2 1 1
+        </DataArray>
+      </Points>
+      <Cells>
+        <DataArray type="Int64" Name="connectivity" format="ascii">0 1 4 3 6 7 10 9  1 2 5 4 7 8 11 10</DataArray>
+        <DataArray type="Int64" Name="offsets" format="ascii">8 16</DataArray>
+        <DataArray type="UInt8" Name="types" format="ascii">12 12</DataArray>
+      </Cells>
+    </Piece>
+  </UnstructuredGrid>
+</VTKFile>
+"""
+
+
+def test_hand_written_ascii_file(tmp_path):
+    p = tmp_path / "h.vtu"
+    p.write_text(ASCII_JOB)
+    v = vtkio.read_vtu(str(p))
+    assert v.points.shape == (12, 3) and v.points[5].tolist() == [2, 1, 0]
+    assert v.cells.tolist() == [[0, 1, 4, 3, 6, 7, 10, 9], [1, 2, 5, 4, 7, 8, 11, 10]]
+    assert v.value("dt") == 2.5e-08
+    f = v.field("density")  # PointData comes first, as the reference reads fields
+    assert f.type == cases.FIELD_NODE and f.data.tolist() == list(range(1, 13))
+    c = v.field("C11")
+    assert c.type == cases.FIELD_CELL and c.data.tolist() == [1.5e11, 2.5e11]
+
+
+def replace_array(xml, name_attr, new_element):
+    start = xml.index(name_attr)
+    start = xml.rindex("<DataArray", 0, start)
+    end = xml.index("</DataArray>", start) + len("</DataArray>")
+    return xml[:start] + new_element + xml[end:]
+
+
+def test_big_endian_inline_binary(tmp_path):
+    # inline binary = base64 of [UInt32 byte count][data], here big-endian as the attribute says
+    vals = [100.0, 200.0]
+    payload = struct.pack(">I", 16) + struct.pack(">2d", *vals)
+    el = '<DataArray type="Float64" Name="density" format="binary">%s</DataArray>' % base64.b64encode(
+        payload).decode()
+    xml = replace_array(ASCII_JOB, 'Name="density" format="ascii">100', el)
+    xml = xml.replace('byte_order="LittleEndian"', 'byte_order="BigEndian"')
+    p = tmp_path / "be.vtu"
+    p.write_text(xml)
+    assert vtkio.read_vtu(str(p)).cell_data["density"].tolist() == vals
+
+
+def test_zlib_block_header_from_the_format(tmp_path):
+    # compressed inline: base64(header) + base64(blocks); header (UInt64 here) =
+    # [number of blocks, uncompressed block size, size of the last block, compressed sizes]
+    data = np.arange(10, dtype=np.float64) * 0.5
+    raw = data.tobytes()
+    blocks = [raw[:48], raw[48:]]
+    comp = [zlib.compress(b) for b in blocks]
+    head = struct.pack("<5Q", 2, 48, len(blocks[1]), len(comp[0]), len(comp[1]))
+    text = (base64.b64encode(head) + base64.b64encode(b"".join(comp))).decode()
+    xml = ASCII_JOB.replace('version="0.1" byte_order="LittleEndian"',
+                            'version="1.0" byte_order="LittleEndian" header_type="UInt64" '
+                            'compressor="vtkZLibDataCompressor"')
+    xml = replace_array(xml, 'Name="dt"', '<DataArray type="Float64" Name="dt" format="binary">%s</DataArray>' % text)
+    p = tmp_path / "z.vtu"
+    p.write_text(xml)
+    assert np.array_equal(vtkio.read_vtu(str(p)).field_data["dt"], data)
+
+
+def test_voxel_cells_are_reordered(tmp_path):
+    xml = ASCII_JOB.replace('<DataArray type="Int64" Name="connectivity" format="ascii">0 1 4 3 6 7 10 9',
+                            '<DataArray type="Int64" Name="connectivity" format="ascii">0 1 3 4 6 7 9 10')
+    xml = xml.replace(">12 12<", ">11 12<")
+    p = tmp_path / "v.vtu"
+    p.write_text(xml)
+    assert vtkio.read_vtu(str(p)).cells[0].tolist() == [0, 1, 4, 3, 6, 7, 10, 9]
+
+
+def test_other_cell_types_are_rejected(tmp_path):
+    p = tmp_path / "t.vtu"
+    p.write_text(ASCII_JOB.replace(">12 12<", ">12 10<"))
+    with pytest.raises(vtkio.VtkError, match="hexahedral"):
+        vtkio.read_vtu(str(p))
+
+
+@pytest.mark.parametrize("fmt", ["appended", "binary"])
+def test_job_round_trip(tmp_path, fmt):
+    case = cases.al_block(seed=8, nx=9, ny=8, nz=5, nsteps=40, n_sensors=3)
+    wt = np.arange(5) * 9.5 * case.dt
+    p = tmp_path / "job.vtu"
+    vtkio.case_to_job(case, str(p), wt, fmt=fmt, compress=(fmt == "binary"))
+    got, wt2 = vtkio.job_case(str(p))
+    assert np.array_equal(wt2, wt) and got.dt == case.dt
+    assert np.array_equal(got.cells, case.cells) and np.array_equal(got.x, case.x) and np.array_equal(got.z, case.z)
+    assert np.array_equal(got.rho.data, case.rho.data)
+    for a, b in zip(got.cij, case.cij):
+        assert np.array_equal(a.data, np.broadcast_to(b.data, a.data.shape))
+    d0, d1 = got.force[0], case.force[0]
+    assert tuple(d0.center) == tuple(d1.center) and d0.radius == d1.radius and d0.ds == d1.ds
+    assert np.array_equal(d0.signal_t, d1.signal_t) and np.array_equal(d0.signal_v, d1.signal_v)
+    # sensors: every node a transducer's window accepts (HostNormalXDZ::checkAt at nodes)
+    acc = [orc.disc_check(d1.center, d1.radius, d1.ds, (case.x[i], case.y[i], case.z[i]))
+           for i in range(case.n_nodes)]
+    assert got.sensors.shape[0] > 0 and np.array_equal(got.sensors, np.nonzero(acc)[0])
+    # the step count follows the driver's FP64 time accumulation (oracle restatement)
+    assert got.nsteps == np.cumsum(orc.schedule(case.dt, wt))[-1]
+
+
+def test_schedule_matches_the_oracle():
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        dt = float(rng.uniform(1e-9, 1e-8))
+        wt = np.cumsum(rng.uniform(0, 40, size=6)) * dt
+        assert np.array_equal(vtkio.schedule_steps(dt, wt), np.cumsum(orc.schedule(dt, wt)))
