@@ -16,7 +16,7 @@ mpirun on this host's cores, on a bounded sample of the same job; fp32 = the mix
 precision form on the same job with its sensor-trace error against FP64; batched = 64
 independent instances (32^3..128^3 cells, 1500 steps, 50-step graph blocks) advanced
 together in FP32, every instance's traces checked against the same batch run in FP64;
-served = a bounded sample of the seeded inspection stream (cracked brackets and plates,
+served = a bounded sample (the first 120 requests) of the seeded inspection stream (cracked brackets and plates,
 0.5M-8M cells, 1000-4000 steps, Poisson arrivals) through the continuous-batching
 server, with solves/hr, p50/p99 latency and the FP64 spot checks.
 """
@@ -123,7 +123,7 @@ def served(rsg, n, rate, seed):
     from pannde_amd import serve
 
     reqs = serve.request_stream(seed=seed, n=n, rate=rate)
-    srv = serve.Server(log=log)
+    srv = serve.Server(prep_threads=10, log=log)
     rep = srv.serve(reqs)
     keys = ("requests", "elapsed_s", "solves_per_hour", "latency_p50_s", "latency_p99_s", "latency_mean_s",
             "gcell_steps_per_s", "gcell_steps_per_s_with_checks", "blocks", "prep_seconds_mean",
@@ -169,8 +169,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-seed", type=int, default=4)
-    ap.add_argument("--serve-requests", type=int, default=60)
-    ap.add_argument("--serve-rate", type=float, default=2.5)
+    ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-seed", type=int, default=9)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
