@@ -132,6 +132,9 @@ typedef struct {
                                   -1: per-entity coefficient arrays (1) */
   int32_t node_classes;        /* rsg_get_tuning only: node materials in use, -1 = per-node arrays */
   int32_t cell_classes;        /* rsg_get_tuning only: cell materials in use, -1 = per-cell arrays */
+  int32_t links_velocity;      /* rsg_get_tuning only: 0 32-bit global links, 1 16-bit block-local
+                                  (LDS), 2 10-bit block-local packed with the key in 16 bytes */
+  int32_t links_stress;        /* rsg_get_tuning only: the same for the stress kernel */
 } rsg_tuning;
 
 /* rsg_create with explicit kernel and layout choices (NULL tuning = defaults). */

@@ -192,10 +192,14 @@ struct rsg_instance {
   int32_t* v_uoff = nullptr;
   int32_t* v_ulist = nullptr;
   uint16_t* v_lconn = nullptr;
+  uint32_t* v_lkey = nullptr;           // [n][4] 10-bit links + key (keys with staged sets <= 1022)
+  std::vector<uint16_t> v_lconn_h;      // host links for packing v_lkey
   int32_t v_maxu = 0;
   int32_t* s_uoff = nullptr;
   int32_t* s_ulist = nullptr;
   uint16_t* s_lconn = nullptr;
+  uint32_t* s_lkey = nullptr;
+  std::vector<uint16_t> s_lconn_h;
   int32_t s_maxu = 0;
   double* node_w = nullptr;
   double* cell_w = nullptr;
@@ -533,6 +537,27 @@ int copy_boxed(rsg_instance* h, void* dst, const std::vector<double>& src,
   return RSG_OK;
 }
 
+// Packs 8 16-bit local links (0xFFFF = none) and the key of every entity into the 16-byte
+// records the packed kernels read: 10 bits per link, 3 links per word, the key in word 3.
+int upload_packed(uint32_t* dst, const std::vector<uint16_t>& lconn, const std::vector<uint32_t>& keys) {
+  const size_t n = keys.size();
+  std::vector<uint32_t> rec(n * 4);
+  for (size_t i = 0; i < n; i++) {
+    uint32_t w[3] = {0, 0, 0};
+    for (int k = 0; k < 8; k++) {
+      const uint16_t l = lconn[i * 8 + k];
+      const uint32_t v = l == 0xFFFF ? rsg::kNoLink : (uint32_t)l;
+      w[k / 3] |= v << (10 * (k % 3));
+    }
+    rec[i * 4] = w[0];
+    rec[i * 4 + 1] = w[1];
+    rec[i * 4 + 2] = w[2];
+    rec[i * 4 + 3] = keys[i];
+  }
+  HIP_TRY(hipMemcpy(dst, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
+  return RSG_OK;
+}
+
 // Builds the special-node / special-cell tables from the channels and sensors, and
 // NaN-boxes the idt / dcx entries of the special entities on the device.
 int build_tables(rsg_instance* h) {
@@ -618,14 +643,14 @@ int build_tables(rsg_instance* h) {
   // class-table form: keys carry the special flag; the special tables carry the class
   std::vector<int32_t> sp_nkey, sp_ckey;
   std::vector<uint32_t> nkeys, ckeys;
-  if (h->nkey) {
+  if (h->n_nmat >= 0) {
     nkeys = h->nkey_h;
     for (auto& b : node_boxes) {
       sp_nkey.push_back((int32_t)h->nkey_h[b.first]);
       nkeys[b.first] = rsg::kSpecialKey | (uint32_t)b.second;
     }
   }
-  if (h->ckey) {
+  if (h->n_cmat >= 0) {
     ckeys = h->ckey_h;
     for (auto& b : cell_boxes) {
       sp_ckey.push_back((int32_t)h->ckey_h[b.first]);
@@ -661,14 +686,20 @@ int build_tables(rsg_instance* h) {
   if ((rc = put_int(h->cell_terms, cterms))) return rc;
   if ((rc = put_int(h->sp_node_key, sp_nkey))) return rc;
   if ((rc = put_int(h->sp_cell_key, sp_ckey))) return rc;
-  if (h->nkey)
+  if (h->v_lkey) {
+    if ((rc = upload_packed(h->v_lkey, h->v_lconn_h, nkeys))) return rc;
+  } else if (h->nkey) {
     HIP_TRY(hipMemcpy(h->nkey, nkeys.data(), nkeys.size() * 4, hipMemcpyHostToDevice));
-  else if ((rc = copy_boxed(h, h->idt, idt, node_boxes)))
+  } else if ((rc = copy_boxed(h, h->idt, idt, node_boxes))) {
     return rc;
-  if (h->ckey)
+  }
+  if (h->s_lkey) {
+    if ((rc = upload_packed(h->s_lkey, h->s_lconn_h, ckeys))) return rc;
+  } else if (h->ckey) {
     HIP_TRY(hipMemcpy(h->ckey, ckeys.data(), ckeys.size() * 4, hipMemcpyHostToDevice));
-  else if ((rc = copy_boxed(h, h->dc[0], dcx, cell_boxes)))
+  } else if ((rc = copy_boxed(h, h->dc[0], dcx, cell_boxes))) {
     return rc;
+  }
   // amplitude staging for kBlockSteps steps
   const size_t nch = std::max<size_t>(1, h->channels.size());
   for (int b = 0; b < 2; b++) {
@@ -725,6 +756,7 @@ void fill_args(rsg_instance* h, rsg::VelocityArgs<T>& va, rsg::StressArgs<T>& sa
   va.sp_sens = h->sp_node_sens.p;
   va.term = h->node_terms.p;
   va.key = h->nkey;
+  va.lkey = h->v_lkey;
   va.cls = (const T*)h->nmat;
   va.axis = (const T*)h->naxis;
   va.sp_key = (const uint32_t*)h->sp_node_key.p;
@@ -755,6 +787,7 @@ void fill_args(rsg_instance* h, rsg::VelocityArgs<T>& va, rsg::StressArgs<T>& sa
   sa.sp_off = h->sp_cell_off.p;
   sa.term = h->cell_terms.p;
   sa.key = h->ckey;
+  sa.lkey = h->s_lkey;
   sa.cls = (const T*)h->cmat;
   sa.axis = (const T*)h->caxis;
   sa.sp_key = (const uint32_t*)h->sp_cell_key.p;
@@ -859,6 +892,8 @@ int rsg_get_tuning(rsg_handle h, rsg_tuning* t) {
   t->coeff_classes = h->classes ? 1 : -1;
   t->node_classes = h->n_nmat;
   t->cell_classes = h->n_cmat;
+  t->links_velocity = h->v_lkey ? 2 : h->v_lconn ? 1 : 0;
+  t->links_stress = h->s_lkey ? 2 : h->s_lconn ? 1 : 0;
   return RSG_OK;
 }
 
@@ -1027,9 +1062,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
                : make_keys<float, 1, 1>(h.get(), nn, entity, h->nkey_h, &h->nmat, &h->naxis, &h->n_nmat);
       if (rc) return rc;
     }
-    if (h->n_nmat >= 0) {
-      if ((rc = h->dalloc((void**)&h->nkey, (size_t)nn * 4))) return rc;
-    } else {
+    if (h->n_nmat < 0) {
       if ((rc = upload_real(h.get(), &h->idt, idt))) return rc;
       for (int d = 0; d < 3; d++)
         if ((rc = upload_real(h.get(), &h->dn[d], dn[d]))) return rc;
@@ -1038,16 +1071,26 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->ncell = (int32_t*)p;
     LdsPlan plan;
+    bool packed = false;
     if (h->lds_v && build_lds_plan(conn, nn, nc, h->block_v * h->cpt_v, plan) &&
         (size_t)6 * plan.max_u * h->elt() <= lds_limit) {
       if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
       h->v_uoff = (int32_t*)p;
       if ((rc = upload(h.get(), &p, plan.ulist))) return rc;
       h->v_ulist = (int32_t*)p;
-      if ((rc = upload(h.get(), &p, plan.lconn))) return rc;
-      h->v_lconn = (uint16_t*)p;
       h->v_maxu = plan.max_u;
+      // with class keys and 10-bit local indices the links and the key share one record
+      packed = h->n_nmat >= 0 && plan.max_u <= rsg::kMaxPackedSet;
+      if (packed) {
+        if ((rc = h->dalloc((void**)&h->v_lkey, (size_t)nn * 16))) return rc;
+        h->v_lconn_h = std::move(plan.lconn);
+      } else {
+        if ((rc = upload(h.get(), &p, plan.lconn))) return rc;
+        h->v_lconn = (uint16_t*)p;
+      }
     }
+    if (h->n_nmat >= 0 && !packed)
+      if ((rc = h->dalloc((void**)&h->nkey, (size_t)nn * 4))) return rc;
     if ((rc = upload(h.get(), &p, w))) return rc;
     h->node_w = (double*)p;
     return RSG_OK;
@@ -1083,9 +1126,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
                : make_keys<float, 21, rsg::kMatRow>(h.get(), nc, entity, h->ckey_h, &h->cmat, &h->caxis, &h->n_cmat);
       if (rc) return rc;
     }
-    if (h->n_cmat >= 0) {
-      if ((rc = h->dalloc((void**)&h->ckey, (size_t)nc * 4))) return rc;
-    } else {
+    if (h->n_cmat < 0) {
       std::vector<double> C((size_t)21 * ncp, 0.0);
       for (int32_t i = 0; i < nc; i++)
         for (int j = 0; j < 21; j++) {
@@ -1100,16 +1141,25 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if ((rc = upload(h.get(), &p, conn))) return rc;
     h->cnode = (int32_t*)p;
     LdsPlan plan;
+    bool packed = false;
     if (h->lds_s && build_lds_plan(conn, nc, nn, h->block_s * h->cpt_s, plan) &&
         (size_t)3 * plan.max_u * h->elt() <= lds_limit) {
       if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
       h->s_uoff = (int32_t*)p;
       if ((rc = upload(h.get(), &p, plan.ulist))) return rc;
       h->s_ulist = (int32_t*)p;
-      if ((rc = upload(h.get(), &p, plan.lconn))) return rc;
-      h->s_lconn = (uint16_t*)p;
       h->s_maxu = plan.max_u;
+      packed = h->n_cmat >= 0 && plan.max_u <= rsg::kMaxPackedSet;
+      if (packed) {
+        if ((rc = h->dalloc((void**)&h->s_lkey, (size_t)nc * 16))) return rc;
+        h->s_lconn_h = std::move(plan.lconn);
+      } else {
+        if ((rc = upload(h.get(), &p, plan.lconn))) return rc;
+        h->s_lconn = (uint16_t*)p;
+      }
     }
+    if (h->n_cmat >= 0 && !packed)
+      if ((rc = h->dalloc((void**)&h->ckey, (size_t)nc * 4))) return rc;
     if ((rc = upload(h.get(), &p, w))) return rc;
     h->cell_w = (double*)p;
     return RSG_OK;
@@ -1504,8 +1554,6 @@ struct rsg_batch_s {
   // kernel configuration shared by every instance
   bool configured = false;
   int block_v = 256, block_s = 256, cpt_v = 1, cpt_s = 1;
-  bool lds_v = true, lds_s = true;
-  int32_t max_u_v = 0, max_u_s = 0;
   // control block on the device: descriptors | velocity tables | stress tables | amplitudes.
   // Each group's table is {n, flags, (instance, block) x n}, 256-byte aligned, with room
   // for n rounded up to kItemQuantum so small changes of membership keep the layout.
@@ -1520,10 +1568,13 @@ struct rsg_batch_s {
   struct Launch {
     int64_t off_v, off_s;
     int32_t grid_v, grid_s;
-    int32_t km_v, km_s;  // coefficient forms (0 arrays, 1 keys, 2 mixed)
+    int32_t km_v, km_s;        // coefficient forms (0 arrays, 1 keys, 3 keys packed with links)
+    bool lds_v, lds_s;         // LDS staging
+    int32_t max_u_v, max_u_s;  // largest staged set of a member
     bool operator==(const Launch& o) const {
       return off_v == o.off_v && off_s == o.off_s && grid_v == o.grid_v && grid_s == o.grid_s &&
-             km_v == o.km_v && km_s == o.km_s;
+             km_v == o.km_v && km_s == o.km_s && lds_v == o.lds_v && lds_s == o.lds_s &&
+             max_u_v == o.max_u_v && max_u_s == o.max_u_s;
     }
   };
   std::vector<Launch> layout, graph_layout;
@@ -1627,16 +1678,16 @@ int batch_launch_block(rsg_batch_s* b) {
         hipError_t e;
         if (sizeof(T) == 8) {
           e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv + g.off_v, k, g.grid_v, b->block_v,
-                                    b->lds_v, b->max_u_v, b->cpt_v, g.km_v, true, b->stream, graph, node);
+                                    g.lds_v, g.max_u_v, b->cpt_v, g.km_v, true, b->stream, graph, node);
           if (e != hipSuccess) return e;
           e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws + g.off_s, k, g.grid_s, b->block_s,
-                                    b->lds_s, b->max_u_s, b->cpt_s, g.km_s, false, b->stream, graph, node);
+                                    g.lds_s, g.max_u_s, b->cpt_s, g.km_s, false, b->stream, graph, node);
         } else {
           e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv + g.off_v, k, g.grid_v, b->block_v,
-                                    b->lds_v, b->max_u_v, b->cpt_v, g.km_v, true, b->stream, graph, node);
+                                    g.lds_v, g.max_u_v, b->cpt_v, g.km_v, true, b->stream, graph, node);
           if (e != hipSuccess) return e;
           e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws + g.off_s, k, g.grid_s, b->block_s,
-                                    b->lds_s, b->max_u_s, b->cpt_s, g.km_s, false, b->stream, graph, node);
+                                    g.lds_s, g.max_u_s, b->cpt_s, g.km_s, false, b->stream, graph, node);
         }
         if (e != hipSuccess) return e;
       }
@@ -1723,63 +1774,86 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
     }
     amp_off += nch * (size_t)b->K;
   }
-  // groups of consecutive active instances, each within the state budget (an instance
-  // larger than the budget forms a group of its own)
+  // groups: instances of one coefficient form per kernel (so each launch runs one kernel
+  // variant), in slot order within the form, each group within the state budget (an
+  // instance larger than the budget forms a group of its own)
   const int32_t flags = b->xcd_run < 0 ? 0 : 1 | (b->xcd_run << 8);
-  // coefficient form of each kernel over the active instances: keys, arrays or both
-  int form_v = 0, form_s = 0;  // bit 0: some instance uses arrays, bit 1: some uses keys
+  auto form = [](const rsg_instance* h, bool velocity) -> int32_t {
+    if (velocity) return h->v_lkey ? 3 : h->nkey ? 1 : 0;
+    return h->s_lkey ? 3 : h->ckey ? 1 : 0;
+  };
+  std::vector<std::pair<int32_t, int32_t>> forms;  // distinct (velocity, stress) forms, first seen first
   for (size_t i = 0; i < b->slots.size(); i++)
     if (kmax[i]) {
-      form_v |= b->slots[i].h->nkey ? 2 : 1;
-      form_s |= b->slots[i].h->ckey ? 2 : 1;
+      const std::pair<int32_t, int32_t> f(form(b->slots[i].h, true), form(b->slots[i].h, false));
+      if (std::find(forms.begin(), forms.end(), f) == forms.end()) forms.push_back(f);
     }
-  const int32_t km_v = form_v == 3 ? 2 : form_v == 2 ? 1 : 0;
-  const int32_t km_s = form_s == 3 ? 2 : form_s == 2 ? 1 : 0;
   const int32_t cv = b->block_v * b->cpt_v, cs = b->block_s * b->cpt_s;
   b->layout.clear();
   size_t ov = 0, os = 0;
   std::vector<size_t> members;
-  for (size_t i = 0; i < b->slots.size();) {
-    members.clear();
-    int64_t bytes = 0;
-    int32_t nv = 0, ns = 0;
-    for (; i < b->slots.size(); i++) {
-      if (!kmax[i]) continue;
-      const rsg_instance* h = b->slots[i].h;
-      const int64_t sb = state_bytes(b, h);
-      if (!members.empty() && b->group_bytes > 0 && bytes + sb > b->group_bytes) break;
-      bytes += sb;
-      members.push_back(i);
-      nv += (h->nn + cv - 1) / cv;
-      ns += (h->nc + cs - 1) / cs;
-    }
-    if (members.empty()) break;
-    if (ov + table_words(nv) > b->cap_wv || os + table_words(ns) > b->cap_ws)
-      return fail(RSG_ERR_STATE, "batch work tables overflow");
-    int32_t* tv = wv + ov;
-    int32_t* ts = ws + os;
-    tv[0] = nv;
-    tv[1] = flags;
-    ts[0] = ns;
-    ts[1] = flags;
-    int32_t iv = 0, is = 0;
-    for (size_t e : members) {
-      const rsg_instance* h = b->slots[e].h;
-      const int32_t nbv = (h->nn + cv - 1) / cv, nbs = (h->nc + cs - 1) / cs;
-      for (int32_t q = 0; q < nbv; q++, iv++) {
-        tv[2 + 2 * iv] = (int32_t)e;
-        tv[3 + 2 * iv] = q;
+  for (const auto& f : forms) {
+    for (size_t i = 0; i < b->slots.size();) {
+      members.clear();
+      int64_t bytes = 0;
+      int32_t nv = 0, ns = 0;
+      for (; i < b->slots.size(); i++) {
+        if (!kmax[i]) continue;
+        const rsg_instance* h = b->slots[i].h;
+        if (form(h, true) != f.first || form(h, false) != f.second) continue;
+        const int64_t sb = state_bytes(b, h);
+        if (!members.empty() && b->group_bytes > 0 && bytes + sb > b->group_bytes) break;
+        bytes += sb;
+        members.push_back(i);
+        nv += (h->nn + cv - 1) / cv;
+        ns += (h->nc + cs - 1) / cs;
       }
-      for (int32_t q = 0; q < nbs; q++, is++) {
-        ts[2 + 2 * is] = (int32_t)e;
-        ts[3 + 2 * is] = q;
+      if (members.empty()) break;
+      if (ov + table_words(nv) > b->cap_wv || os + table_words(ns) > b->cap_ws)
+        return fail(RSG_ERR_STATE, "batch work tables overflow");
+      int32_t* tv = wv + ov;
+      int32_t* ts = ws + os;
+      tv[0] = nv;
+      tv[1] = flags;
+      ts[0] = ns;
+      ts[1] = flags;
+      int32_t iv = 0, is = 0;
+      // LDS staging for a kernel when every member has a plan, sized for the largest set
+      bool lds_v = true, lds_s = true;
+      int32_t mu_v = 0, mu_s = 0;
+      for (size_t e : members) {
+        const rsg_instance* h = b->slots[e].h;
+        lds_v = lds_v && (h->v_lconn || h->v_lkey);
+        lds_s = lds_s && (h->s_lconn || h->s_lkey);
+        mu_v = std::max(mu_v, h->v_maxu);
+        mu_s = std::max(mu_s, h->s_maxu);
+        const int32_t nbv = (h->nn + cv - 1) / cv, nbs = (h->nc + cs - 1) / cs;
+        for (int32_t q = 0; q < nbv; q++, iv++) {
+          tv[2 + 2 * iv] = (int32_t)e;
+          tv[3 + 2 * iv] = q;
+        }
+        for (int32_t q = 0; q < nbs; q++, is++) {
+          ts[2 + 2 * is] = (int32_t)e;
+          ts[3 + 2 * is] = q;
+        }
       }
+      const int32_t qv = (nv + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
+      const int32_t qs = (ns + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
+      rsg_batch_s::Launch g;
+      g.off_v = (int64_t)ov;
+      g.off_s = (int64_t)os;
+      g.grid_v = b->dense ? qv : b->grid;
+      g.grid_s = b->dense ? qs : b->grid;
+      g.km_v = f.first;
+      g.km_s = f.second;
+      g.lds_v = lds_v;
+      g.lds_s = lds_s;
+      g.max_u_v = lds_v ? mu_v : 0;
+      g.max_u_s = lds_s ? mu_s : 0;
+      b->layout.push_back(g);
+      ov += table_words(nv);
+      os += table_words(ns);
     }
-    const int32_t qv = (nv + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
-    const int32_t qs = (ns + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
-    b->layout.push_back({(int64_t)ov, (int64_t)os, b->dense ? qv : b->grid, b->dense ? qs : b->grid, km_v, km_s});
-    ov += table_words(nv);
-    os += table_words(ns);
   }
   if (b->layout.empty()) {
     *unfinished = 0;
@@ -1852,23 +1926,12 @@ int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot) {
     b->block_s = h->block_s;
     b->cpt_v = h->cpt_v;
     b->cpt_s = h->cpt_s;
-    b->lds_v = h->v_lconn != nullptr;
-    b->lds_s = h->s_lconn != nullptr;
     b->configured = true;
   }
   int rc;
   if (h->tables_dirty && (rc = build_tables(h))) return rc;
   // the handle's own uploads must be complete; blocks already enqueued do not involve it
   HIP_TRY(hipStreamSynchronize(h->stream));
-  // one kernel variant serves the batch: LDS staging only if every instance has a plan,
-  // sized for the largest staged set
-  const bool lv = b->lds_v && h->v_lconn, ls = b->lds_s && h->s_lconn;
-  const int32_t mv = std::max(b->max_u_v, h->v_maxu), ms = std::max(b->max_u_s, h->s_maxu);
-  if (lv != b->lds_v || ls != b->lds_s || mv != b->max_u_v || ms != b->max_u_s) b->graph_dirty = true;
-  b->lds_v = lv;
-  b->lds_s = ls;
-  b->max_u_v = mv;
-  b->max_u_s = ms;
   int32_t id = -1;
   for (size_t i = 0; i < b->slots.size(); i++)
     if (!b->slots[i].attached) {

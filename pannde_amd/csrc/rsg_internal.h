@@ -29,6 +29,8 @@ constexpr int kMatShift = 21;
 constexpr int kMaxMaterials = 1024;
 constexpr int kAxisClasses = 128;  // spacing classes per axis; tables are [3][kAxisClasses]
 constexpr int kMatRow = 24;        // cell material row: the 21 C as the C array holds them, padded
+constexpr uint32_t kNoLink = 0x3FF;  // packed 10-bit link: no neighbour
+constexpr int kMaxPackedSet = 1022;  // largest staged set whose local indices fit 10 bits
 
 inline uint32_t make_key(int mat, int ix, int iy, int iz) {
   return ((uint32_t)mat << kMatShift) | ((uint32_t)ix << 14) | ((uint32_t)iy << 7) | (uint32_t)iz;
@@ -64,11 +66,15 @@ struct VelocityArgs {
   const int32_t* term;     // channel id per term
   const double* amp;       // [n_channels] amplitudes of this step
   double* trace;           // [n_sensors][3] row of this step
-  // class tables (key != nullptr: idt, dnx..dnz and sp_idt are not used)
+  // class tables (key or lkey != nullptr: idt, dnx..dnz and sp_idt are not used)
   const uint32_t* key;     // [n] material/spacing key, or kSpecialKey | special index
   const T* cls;            // [materials] dt/rho
   const T* axis;           // [3][kAxisClasses] node spacings as dnx..dnz hold them
   const uint32_t* sp_key;  // [nsp] key of each special node
+  // LDS links and key in one 16-byte record (lkey != nullptr: lconn and key are not used):
+  // words 0..2 hold the 8 block-local indices, 10 bits each (3 per word, kNoLink = none),
+  // word 3 the key
+  const uint32_t* lkey;    // [n][4]
 };
 
 template <typename T>
@@ -101,11 +107,12 @@ struct StressArgs {
   const int32_t* sp_off;  // [nsp][7] term ranges for Voigt components 0..5
   const int32_t* term;
   const double* amp;
-  // class tables (key != nullptr: C, dcx..dcz and sp_dcx are not used)
+  // class tables (key or lkey != nullptr: C, dcx..dcz and sp_dcx are not used)
   const uint32_t* key;     // [n] material/spacing key, or kSpecialKey | special index
   const T* cls;            // [materials][kMatRow]
   const T* axis;           // [3][kAxisClasses] cell spacings as dcx..dcz hold them
   const uint32_t* sp_key;  // [nsp] key of each special cell
+  const uint32_t* lkey;    // [n][4] LDS links + key, as for VelocityArgs
 };
 
 // Energy reduction arguments (FP64 accumulation, one partial per block).
@@ -144,8 +151,8 @@ hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t
 hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t s);
 // Batched launch: grid workgroups walk work = {n, flags, (instance, block)...}; lds
 // selects LDS staging (max_u: the largest staged set of any instance), cpt the entities
-// per thread (1, 2 or 4), km the coefficient form (0 per-entity arrays, 1 class-table
-// keys, 2 per instance).  With graph != nullptr the launch is added to the graph as a
+// per thread (1, 2 or 4), km the coefficient form of every instance the table names (0
+// per-entity arrays, 1 class-table keys, 3 keys packed with the LDS links).  With graph != nullptr the launch is added to the graph as a
 // kernel node that depends on *node (when not null) and *node becomes the new node;
 // otherwise it is enqueued on s.
 hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* work, int k, int grid, int block,

@@ -161,10 +161,14 @@ __device__ __forceinline__ void load_row(const float* __restrict__ p, float (&ou
 }
 
 // Coefficient form of a kernel instance: KM = 0 per-entity arrays, 1 class-table keys,
-// 2 either, chosen per instance by a.key (batches that mix the two forms).
+// 3 keys packed with the LDS links (a batch launches each form for its own members).
 template <int KM, typename A>
-__device__ __forceinline__ bool keyed(const A& a) {
-  return KM == 1 || (KM == 2 && a.key != nullptr);
+__device__ __forceinline__ constexpr bool packed(const A&) {
+  return KM == 3;
+}
+template <int KM, typename A>
+__device__ __forceinline__ constexpr bool keyed(const A&) {
+  return KM == 1 || KM == 3;
 }
 
 // Material/spacing key of an entity, and its special index (or -1).
@@ -203,6 +207,18 @@ __device__ __forceinline__ void load_lconn(const uint16_t* __restrict__ lconn, i
     c[2 * k] = lo == 0xFFFF ? -1 : lo;
     c[2 * k + 1] = hi == 0xFFFF ? -1 : hi;
   }
+}
+
+// 8 block-local indices (10 bits each, kNoLink = none) and the key of one entity, one
+// 16-byte load.
+__device__ __forceinline__ uint32_t load_lkey(const uint32_t* __restrict__ lkey, int i, int (&c)[8]) {
+  const v4i w = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(lkey) + (size_t)i);
+  const uint32_t w0 = (uint32_t)w.x, w1 = (uint32_t)w.y, w2 = (uint32_t)w.z;
+  const uint32_t l[8] = {w0 & kNoLink, (w0 >> 10) & kNoLink, (w0 >> 20) & kNoLink, w1 & kNoLink,
+                         (w1 >> 10) & kNoLink, (w1 >> 20) & kNoLink, w2 & kNoLink, (w2 >> 10) & kNoLink};
+#pragma unroll
+  for (int k = 0; k < 8; k++) c[k] = l[k] == kNoLink ? -1 : (int)l[k];
+  return (uint32_t)w.w;
 }
 
 // Stages NCOMP fields of the block's unique gather set (ulist[u_off[b]..u_off[b+1]))
@@ -253,12 +269,16 @@ struct VelIn {
 
 template <typename T, bool LDS, int KM>
 __device__ __forceinline__ void velocity_load(const VelocityArgs<T>& a, int n, VelIn<T>& in) {
-  if constexpr (LDS)
-    load_lconn(a.lconn, n, in.c);
-  else
+  if constexpr (LDS) {
+    if (packed<KM>(a))
+      in.key = load_lkey(a.lkey, n, in.c);
+    else
+      load_lconn(a.lconn, n, in.c);
+  } else {
     load_conn(a.ncell, n, in.c);
+  }
   if (keyed<KM>(a)) {
-    in.key = __builtin_nontemporal_load(a.key + n);
+    if (!(LDS && packed<KM>(a))) in.key = __builtin_nontemporal_load(a.key + n);
     in.idt = in.sx = in.sy = in.sz = T(0);
   } else {
     in.key = 0;
@@ -408,12 +428,16 @@ struct StrIn {
 
 template <typename T, bool LDS, int KM>
 __device__ __forceinline__ void stress_load(const StressArgs<T>& a, int e, StrIn<T>& in) {
-  if constexpr (LDS)
-    load_lconn(a.lconn, e, in.c);
-  else
+  if constexpr (LDS) {
+    if (packed<KM>(a))
+      in.key = load_lkey(a.lkey, e, in.c);
+    else
+      load_lconn(a.lconn, e, in.c);
+  } else {
     load_conn(a.cnode, e, in.c);
+  }
   if (keyed<KM>(a)) {
-    in.key = __builtin_nontemporal_load(a.key + e);
+    if (!(LDS && packed<KM>(a))) in.key = __builtin_nontemporal_load(a.key + e);
     in.sx = in.sy = in.sz = T(0);
 #pragma unroll
     for (int j = 0; j < 21; j++) in.cc[j] = T(0);
@@ -648,7 +672,12 @@ hipError_t launch1d(K kernel, const Args& a, int n, int block, size_t lds, hipSt
 template <typename F>
 hipError_t with_variant(bool lds, int max_u, int block, int cpt, int km, F&& f) {
   auto by_u = [&](auto cptc, auto kmc) -> hipError_t {
-    if (!lds) return f(std::false_type(), std::integral_constant<int, 0>(), cptc, kmc);
+    if (!lds) {
+      if constexpr (decltype(kmc)::value == 3)
+        return hipErrorInvalidValue;  // packed links exist only with LDS staging
+      else
+        return f(std::false_type(), std::integral_constant<int, 0>(), cptc, kmc);
+    }
     switch ((max_u + block - 1) / block) {
       case 0:
       case 1: return f(std::true_type(), std::integral_constant<int, 1>(), cptc, kmc);
@@ -669,38 +698,15 @@ hipError_t with_variant(bool lds, int max_u, int block, int cpt, int km, F&& f) 
   switch (km) {
     case 0: return by_cpt(std::integral_constant<int, 0>());
     case 1: return by_cpt(std::integral_constant<int, 1>());
-    default: return hipErrorInvalidValue;
-  }
-}
-
-// The batched kernels also come in the mixed form (KM = 2).
-template <typename F>
-hipError_t with_batch_variant(bool lds, int max_u, int block, int cpt, int km, F&& f) {
-  if (km != 2) return with_variant(lds, max_u, block, cpt, km, f);
-  auto by_u = [&](auto cptc) -> hipError_t {
-    constexpr std::integral_constant<int, 2> kmc;
-    if (!lds) return f(std::false_type(), std::integral_constant<int, 0>(), cptc, kmc);
-    switch ((max_u + block - 1) / block) {
-      case 0:
-      case 1: return f(std::true_type(), std::integral_constant<int, 1>(), cptc, kmc);
-      case 2: return f(std::true_type(), std::integral_constant<int, 2>(), cptc, kmc);
-      case 3: return f(std::true_type(), std::integral_constant<int, 3>(), cptc, kmc);
-      case 4: return f(std::true_type(), std::integral_constant<int, 4>(), cptc, kmc);
-      default: return f(std::true_type(), std::integral_constant<int, 0>(), cptc, kmc);
-    }
-  };
-  switch (cpt) {
-    case 1: return by_u(std::integral_constant<int, 1>());
-    case 2: return by_u(std::integral_constant<int, 2>());
-    case 4: return by_u(std::integral_constant<int, 4>());
+    case 3: return by_cpt(std::integral_constant<int, 3>());
     default: return hipErrorInvalidValue;
   }
 }
 
 template <typename T>
 hipError_t launch_velocity(const VelocityArgs<T>& a, int block, hipStream_t s) {
-  const bool lds = a.lconn != nullptr;
-  return with_variant(lds, a.max_u, block, a.cpt, a.key ? 1 : 0, [&](auto L, auto U, auto C, auto K) {
+  const bool lds = a.lconn != nullptr || a.lkey != nullptr;
+  return with_variant(lds, a.max_u, block, a.cpt, a.lkey ? 3 : a.key ? 1 : 0, [&](auto L, auto U, auto C, auto K) {
     return launch1d(velocity_kernel<T, decltype(L)::value, decltype(U)::value, decltype(C)::value, decltype(K)::value>,
                     a, a.n, block, lds ? (size_t)6 * a.max_u * sizeof(T) : 0, s);
   });
@@ -708,8 +714,8 @@ hipError_t launch_velocity(const VelocityArgs<T>& a, int block, hipStream_t s) {
 
 template <typename T>
 hipError_t launch_stress(const StressArgs<T>& a, int block, hipStream_t s) {
-  const bool lds = a.lconn != nullptr;
-  return with_variant(lds, a.max_u, block, a.cpt, a.key ? 1 : 0, [&](auto L, auto U, auto C, auto K) {
+  const bool lds = a.lconn != nullptr || a.lkey != nullptr;
+  return with_variant(lds, a.max_u, block, a.cpt, a.lkey ? 3 : a.key ? 1 : 0, [&](auto L, auto U, auto C, auto K) {
     return launch1d(stress_kernel<T, decltype(L)::value, decltype(U)::value, decltype(C)::value, decltype(K)::value>,
                     a, a.n, block, lds ? (size_t)3 * a.max_u * sizeof(T) : 0, s);
   });
@@ -736,7 +742,7 @@ hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* work, int k, in
   if (grid <= 0) return hipSuccess;
   if (block < 64 || block > RSG_MAX_BLOCK || (block % 64) != 0) return hipErrorInvalidValue;
   const size_t bytes = lds ? (size_t)(velocity ? 6 : 3) * max_u * sizeof(T) : 0;
-  return with_batch_variant(lds, max_u, block, cpt, km, [&](auto L, auto U, auto C, auto K) {
+  return with_variant(lds, max_u, block, cpt, km, [&](auto L, auto U, auto C, auto K) {
     constexpr bool l = decltype(L)::value;
     constexpr int u = decltype(U)::value, c = decltype(C)::value, m = decltype(K)::value;
     if (graph) {

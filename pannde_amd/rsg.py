@@ -66,7 +66,8 @@ class TuningDesc(ctypes.Structure):
                 ("lds_velocity", ctypes.c_int32), ("lds_stress", ctypes.c_int32), ("tile", ctypes.c_int32 * 3),
                 ("c_layout", ctypes.c_int32),
                 ("xcd_remap", ctypes.c_int32), ("coeff_classes", ctypes.c_int32),
-                ("node_classes", ctypes.c_int32), ("cell_classes", ctypes.c_int32)]
+                ("node_classes", ctypes.c_int32), ("cell_classes", ctypes.c_int32),
+                ("links_velocity", ctypes.c_int32), ("links_stress", ctypes.c_int32)]
 
 
 def tuning_desc(tuning):
@@ -241,7 +242,8 @@ class DeviceSolver:
                     per_thread_velocity=t.per_thread_velocity, per_thread_stress=t.per_thread_stress,
                     lds_velocity=t.lds_velocity, lds_stress=t.lds_stress, tile=tuple(t.tile),
                     c_layout=t.c_layout, xcd_remap=t.xcd_remap, coeff_classes=t.coeff_classes,
-                    node_classes=t.node_classes, cell_classes=t.cell_classes)
+                    node_classes=t.node_classes, cell_classes=t.cell_classes,
+                    links_velocity=t.links_velocity, links_stress=t.links_stress)
 
     # -- sources / sensors
     def set_sources(self, force=(), moment=()):

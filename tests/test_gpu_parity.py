@@ -114,6 +114,27 @@ def test_coefficient_class_tables_in_use():
     assert np.array_equal(dev[0], ref[0]) and np.array_equal(dev[1], ref[1])
 
 
+FORMS = [  # (tuning, expected velocity links, expected stress links)
+    (dict(), 2, 2),                                          # keys packed with 10-bit links
+    (dict(coeff_classes=-1), 1, 1),                          # per-entity arrays, 16-bit links
+    (dict(lds_velocity=-1, lds_stress=-1), 0, 0),            # keys, direct gathers
+    (dict(block_stress=512, per_thread_stress=2), 2, 1),     # staged sets above 1022: keys + 16-bit
+]
+
+
+@pytest.mark.parametrize("form", FORMS, ids=[str(f[0]) for f in FORMS])
+def test_link_and_coefficient_forms_agree(form):
+    tuning, lv, ls = form
+    case = cases.al_block(seed=9, nx=40, ny=40, nz=20, nsteps=60, n_sensors=6)
+    s = rsg.DeviceSolver.from_case(case, tuning=tuning)
+    t = s.tuning()
+    s.solve(case.nsteps)
+    dev = (*s.get_state(), s.time, s.traces())
+    s.close()
+    assert (t["links_velocity"], t["links_stress"]) == (lv, ls)
+    assert_same(dev, orc.run_oracle(case))
+
+
 @pytest.mark.parametrize("tuning", [dict(block_stress=1024), dict(block_velocity=96), dict(per_thread_stress=3)])
 def test_unsupported_tuning_is_rejected(tuning):
     case, _ = load_golden("al_block_cellrho")
