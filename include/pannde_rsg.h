@@ -119,7 +119,9 @@ typedef struct {
   int32_t block_velocity;      /* workgroup size of the velocity kernel, 64..512 step 64 (256) */
   int32_t block_stress;        /* workgroup size of the stress kernel (256) */
   int32_t per_thread_velocity; /* nodes per thread: 1, 2 or 4 (1) */
-  int32_t per_thread_stress;   /* cells per thread: 1, 2 or 4 (1) */
+  int32_t per_thread_stress;   /* cells per thread: 1, 2 or 4 (2 for meshes of 3 x 2^20 cells
+                                  or more, else 1: measured on a 3.4 M-cell plate and on
+                                  32^3..128^3 blocks) */
   int32_t lds_velocity;        /* 1: stage each workgroup's gathers in LDS, -1: gather from HBM (1) */
   int32_t lds_stress;          /* the same for the stress kernel (1) */
   int32_t tile[3];             /* locality brick in cells along x, y, z (8, 8, 4) */
@@ -213,9 +215,9 @@ int rsg_last_error(char* buf, size_t len);
  * kernel writes is still cached when the other reads it.  Instances join and leave between K-step blocks; each
  * takes exactly its own number of steps.  No counterpart in the reference, which runs
  * one model per process; each instance's arithmetic is that of rsg_step, so its results
- * are bit-identical alone and in any mix.  Instances are ordinary handles (same
- * precision, same workgroup sizes and per-thread counts), their channels must carry
- * signal tables, and while in a batch they are stepped only by the batch; read their
+ * are bit-identical alone and in any mix.  Instances are ordinary handles of the batch's
+ * precision, their channels must carry signal tables, and while in a batch they are
+ * stepped only by the batch; read their
  * state and traces with rsg_get_state / rsg_get_traces after rsg_batch_synchronize. */
 typedef struct rsg_batch_s* rsg_batch;
 
@@ -227,7 +229,8 @@ int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, i
                      rsg_batch* out);
 
 /* Adds a handle that will take nsteps more steps; *slot receives its slot id.  It joins at
- * the next K-step block; blocks already enqueued keep running. */
+ * the next K-step block; blocks already enqueued keep running.  Members may differ in
+ * their tuning: each launch covers the members of one kernel configuration. */
 int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot);
 
 /* Detaches a slot's handle (finished or not); the handle can be stepped alone again.

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -935,6 +936,10 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
       if (t.tile[d]) h->tile[d] = t.tile[d];
     }
   }
+  // two cells per thread for the stress kernel on meshes of 3 Mi cells or more (their
+  // streaming operands are requested one cell ahead: -9 % stress time on a 3.4 M-cell plate),
+  // one below (measured faster on 32^3..128^3 instances), unless the caller chose
+  if (!tuning || tuning->per_thread_stress == 0) h->cpt_s = nc >= (3 << 20) ? 2 : 1;
   for (int blk : {h->block_v, h->block_s})
     if (blk < 64 || blk > 512 || blk % 64 != 0)
       return fail(RSG_ERR_ARG, "workgroup size %d: must be a multiple of 64 in [64, 512]", blk);
@@ -1551,9 +1556,6 @@ struct rsg_batch_s {
     bool fin_recorded = false;
   };
   std::vector<Slot> slots;
-  // kernel configuration shared by every instance
-  bool configured = false;
-  int block_v = 256, block_s = 256, cpt_v = 1, cpt_s = 1;
   // control block on the device: descriptors | velocity tables | stress tables | amplitudes.
   // Each group's table is {n, flags, (instance, block) x n}, 256-byte aligned, with room
   // for n rounded up to kItemQuantum so small changes of membership keep the layout.
@@ -1571,10 +1573,12 @@ struct rsg_batch_s {
     int32_t km_v, km_s;        // coefficient forms (0 arrays, 1 keys, 3 keys packed with links)
     bool lds_v, lds_s;         // LDS staging
     int32_t max_u_v, max_u_s;  // largest staged set of a member
+    int32_t block_v, block_s, cpt_v, cpt_s;  // the members' workgroup sizes and per-thread counts
     bool operator==(const Launch& o) const {
       return off_v == o.off_v && off_s == o.off_s && grid_v == o.grid_v && grid_s == o.grid_s &&
              km_v == o.km_v && km_s == o.km_s && lds_v == o.lds_v && lds_s == o.lds_s &&
-             max_u_v == o.max_u_v && max_u_s == o.max_u_s;
+             max_u_v == o.max_u_v && max_u_s == o.max_u_s && block_v == o.block_v && block_s == o.block_s &&
+             cpt_v == o.cpt_v && cpt_s == o.cpt_s;
     }
   };
   std::vector<Launch> layout, graph_layout;
@@ -1631,8 +1635,8 @@ int batch_reserve(rsg_batch_s* b) {
   for (auto& sl : b->slots) {
     if (!sl.attached) continue;
     rsg_instance* h = sl.h;
-    nwv += (size_t)(h->nn + b->block_v * b->cpt_v - 1) / (b->block_v * b->cpt_v);
-    nws += (size_t)(h->nc + b->block_s * b->cpt_s - 1) / (b->block_s * b->cpt_s);
+    nwv += (size_t)(h->nn + h->block_v * h->cpt_v - 1) / (h->block_v * h->cpt_v);
+    nws += (size_t)(h->nc + h->block_s * h->cpt_s - 1) / (h->block_s * h->cpt_s);
     namp += std::max<size_t>(1, h->channels.size()) * (size_t)b->K;
   }
   // every group table wastes at most 2 + 2 * (kItemQuantum - 1) + 63 words; there are at
@@ -1677,17 +1681,17 @@ int batch_launch_block(rsg_batch_s* b) {
       for (int k = 0; k < b->K; k++) {
         hipError_t e;
         if (sizeof(T) == 8) {
-          e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv + g.off_v, k, g.grid_v, b->block_v,
-                                    g.lds_v, g.max_u_v, b->cpt_v, g.km_v, true, b->stream, graph, node);
+          e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv + g.off_v, k, g.grid_v, g.block_v,
+                                    g.lds_v, g.max_u_v, g.cpt_v, g.km_v, true, b->stream, graph, node);
           if (e != hipSuccess) return e;
-          e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws + g.off_s, k, g.grid_s, b->block_s,
-                                    g.lds_s, g.max_u_s, b->cpt_s, g.km_s, false, b->stream, graph, node);
+          e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws + g.off_s, k, g.grid_s, g.block_s,
+                                    g.lds_s, g.max_u_s, g.cpt_s, g.km_s, false, b->stream, graph, node);
         } else {
-          e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv + g.off_v, k, g.grid_v, b->block_v,
-                                    g.lds_v, g.max_u_v, b->cpt_v, g.km_v, true, b->stream, graph, node);
+          e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv + g.off_v, k, g.grid_v, g.block_v,
+                                    g.lds_v, g.max_u_v, g.cpt_v, g.km_v, true, b->stream, graph, node);
           if (e != hipSuccess) return e;
-          e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws + g.off_s, k, g.grid_s, b->block_s,
-                                    g.lds_s, g.max_u_s, b->cpt_s, g.km_s, false, b->stream, graph, node);
+          e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws + g.off_s, k, g.grid_s, g.block_s,
+                                    g.lds_s, g.max_u_s, g.cpt_s, g.km_s, false, b->stream, graph, node);
         }
         if (e != hipSuccess) return e;
       }
@@ -1774,25 +1778,26 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
     }
     amp_off += nch * (size_t)b->K;
   }
-  // groups: instances of one coefficient form per kernel (so each launch runs one kernel
-  // variant), in slot order within the form, each group within the state budget (an
-  // instance larger than the budget forms a group of its own)
+  // groups: instances of one launch configuration (coefficient form, workgroup size and
+  // per-thread count of each kernel, so each launch runs one kernel variant), in slot order
+  // within a configuration, each group within the state budget (an instance larger than the
+  // budget forms a group of its own)
   const int32_t flags = b->xcd_run < 0 ? 0 : 1 | (b->xcd_run << 8);
-  auto form = [](const rsg_instance* h, bool velocity) -> int32_t {
-    if (velocity) return h->v_lkey ? 3 : h->nkey ? 1 : 0;
-    return h->s_lkey ? 3 : h->ckey ? 1 : 0;
+  auto config = [](const rsg_instance* h) -> std::array<int32_t, 6> {
+    return {h->v_lkey ? 3 : h->nkey ? 1 : 0, h->s_lkey ? 3 : h->ckey ? 1 : 0, h->block_v, h->block_s, h->cpt_v,
+            h->cpt_s};
   };
-  std::vector<std::pair<int32_t, int32_t>> forms;  // distinct (velocity, stress) forms, first seen first
+  std::vector<std::array<int32_t, 6>> configs;  // distinct configurations, first seen first
   for (size_t i = 0; i < b->slots.size(); i++)
     if (kmax[i]) {
-      const std::pair<int32_t, int32_t> f(form(b->slots[i].h, true), form(b->slots[i].h, false));
-      if (std::find(forms.begin(), forms.end(), f) == forms.end()) forms.push_back(f);
+      const auto f = config(b->slots[i].h);
+      if (std::find(configs.begin(), configs.end(), f) == configs.end()) configs.push_back(f);
     }
-  const int32_t cv = b->block_v * b->cpt_v, cs = b->block_s * b->cpt_s;
   b->layout.clear();
   size_t ov = 0, os = 0;
   std::vector<size_t> members;
-  for (const auto& f : forms) {
+  for (const auto& f : configs) {
+    const int32_t cv = f[2] * f[4], cs = f[3] * f[5];
     for (size_t i = 0; i < b->slots.size();) {
       members.clear();
       int64_t bytes = 0;
@@ -1800,7 +1805,7 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
       for (; i < b->slots.size(); i++) {
         if (!kmax[i]) continue;
         const rsg_instance* h = b->slots[i].h;
-        if (form(h, true) != f.first || form(h, false) != f.second) continue;
+        if (config(h) != f) continue;
         const int64_t sb = state_bytes(b, h);
         if (!members.empty() && b->group_bytes > 0 && bytes + sb > b->group_bytes) break;
         bytes += sb;
@@ -1844,8 +1849,12 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
       g.off_s = (int64_t)os;
       g.grid_v = b->dense ? qv : b->grid;
       g.grid_s = b->dense ? qs : b->grid;
-      g.km_v = f.first;
-      g.km_s = f.second;
+      g.km_v = f[0];
+      g.km_s = f[1];
+      g.block_v = f[2];
+      g.block_s = f[3];
+      g.cpt_v = f[4];
+      g.cpt_s = f[5];
       g.lds_v = lds_v;
       g.lds_s = lds_s;
       g.max_u_v = lds_v ? mu_v : 0;
@@ -1918,16 +1927,6 @@ int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot) {
   if (h->precision != b->precision) return fail(RSG_ERR_ARG, "precision differs from the batch's");
   for (auto& c : h->channels)
     if (!c.has_signal) return fail(RSG_ERR_ARG, "batched instances need signal tables on every channel");
-  if (b->configured) {
-    if (h->block_v != b->block_v || h->block_s != b->block_s || h->cpt_v != b->cpt_v || h->cpt_s != b->cpt_s)
-      return fail(RSG_ERR_ARG, "workgroup sizes / per-thread counts differ from the batch's");
-  } else {
-    b->block_v = h->block_v;
-    b->block_s = h->block_s;
-    b->cpt_v = h->cpt_v;
-    b->cpt_s = h->cpt_s;
-    b->configured = true;
-  }
   int rc;
   if (h->tables_dirty && (rc = build_tables(h))) return rc;
   // the handle's own uploads must be complete; blocks already enqueued do not involve it

@@ -367,13 +367,15 @@ __device__ __forceinline__ void velocity_apply(const VelocityArgs<T>& a, int n, 
 
 // One workgroup's share of the velocity update: chunk b of the instance described by a
 // (CPT * blockDim.x nodes, thread t taking nodes t, t + blockDim.x, ...).  With LDS the
-// chunk's unique cells are staged first; the first node's streaming operands are
-// requested before the staging so their latency overlaps it.
+// chunk's unique cells are staged first; the streaming operands of the first two nodes
+// are requested before the staging so their latency overlaps it.
 template <typename T, bool LDS, int U, int CPT, int KM>
 __device__ __forceinline__ void velocity_body(const VelocityArgs<T>& a, int b, unsigned char* smem) {
-  const int base = b * CPT * (int)blockDim.x + (int)threadIdx.x;
-  VelIn<T> in;
+  const int bd = (int)blockDim.x;
+  const int base = b * CPT * bd + (int)threadIdx.x;
+  VelIn<T> in, next;
   velocity_load<T, LDS, KM>(a, base < a.n ? base : a.n - 1, in);  // tail threads load a valid node
+  if constexpr (CPT > 1) velocity_load<T, LDS, KM>(a, base + bd < a.n ? base + bd : a.n - 1, next);
   const T* S0 = a.S0;
   const T* S1 = a.S1;
   const T* S2 = a.S2;
@@ -393,11 +395,13 @@ __device__ __forceinline__ void velocity_body(const VelocityArgs<T>& a, int b, u
   }
   if (base >= a.n) return;
   velocity_apply<T, KM>(a, base, in, S0, S1, S2, S3, S4, S5);
+  // later nodes: each node's streaming operands are requested one node ahead
 #pragma unroll 1
   for (int j = 1; j < CPT; j++) {
-    const int n = base + j * (int)blockDim.x;
+    const int n = base + j * bd;
     if (n >= a.n) break;
-    velocity_load<T, LDS, KM>(a, n, in);
+    in = next;
+    if (j + 1 < CPT) velocity_load<T, LDS, KM>(a, n + bd < a.n ? n + bd : a.n - 1, next);
     velocity_apply<T, KM>(a, n, in, S0, S1, S2, S3, S4, S5);
   }
 }
@@ -537,9 +541,11 @@ __device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrI
 // overlaps the LDS staging of the chunk's unique nodes.
 template <typename T, bool LDS, int U, int CPT, int KM>
 __device__ __forceinline__ void stress_body(const StressArgs<T>& a, int b, unsigned char* smem) {
-  const int base = b * CPT * (int)blockDim.x + (int)threadIdx.x;
-  StrIn<T> in;
+  const int bd = (int)blockDim.x;
+  const int base = b * CPT * bd + (int)threadIdx.x;
+  StrIn<T> in, next;
   stress_load<T, LDS, KM>(a, base < a.n ? base : a.n - 1, in);  // tail threads load a valid cell
+  if constexpr (CPT > 1) stress_load<T, LDS, KM>(a, base + bd < a.n ? base + bd : a.n - 1, next);
   const T* V0 = a.V0;
   const T* V1 = a.V1;
   const T* V2 = a.V2;
@@ -553,11 +559,13 @@ __device__ __forceinline__ void stress_body(const StressArgs<T>& a, int b, unsig
   }
   if (base >= a.n) return;
   stress_apply<T, KM>(a, base, in, V0, V1, V2);
+  // later cells: each cell's streaming operands are requested one cell ahead
 #pragma unroll 1
   for (int j = 1; j < CPT; j++) {
-    const int e = base + j * (int)blockDim.x;
+    const int e = base + j * bd;
     if (e >= a.n) break;
-    stress_load<T, LDS, KM>(a, e, in);
+    in = next;
+    if (j + 1 < CPT) stress_load<T, LDS, KM>(a, e + bd < a.n ? e + bd : a.n - 1, next);
     stress_apply<T, KM>(a, e, in, V0, V1, V2);
   }
 }

@@ -155,11 +155,30 @@ def test_batched_handle_refuses_direct_use():
     s.solve(1)
 
 
-def test_batch_rejects_mismatched_members():
+def test_batch_rejects_other_precision():
     case, _ = load_golden("al_block_cellrho")
     b = rsg.Batch(rsg.RSG_FP64)
     b.add(rsg.DeviceSolver.from_case(case), 5)
     with pytest.raises(rsg.RSGError, match="precision"):
         b.add(rsg.DeviceSolver.from_case(case, rsg.RSG_FP32), 5)
-    with pytest.raises(rsg.RSGError, match="workgroup"):
-        b.add(rsg.DeviceSolver.from_case(case, tuning=dict(block_stress=128)), 5)
+
+
+TUNED = [None, dict(block_stress=128), dict(per_thread_stress=2, per_thread_velocity=4, block_velocity=128),
+         dict(lds_velocity=-1), dict(coeff_classes=-1, block_velocity=512), dict(per_thread_stress=4)]
+
+
+@pytest.mark.parametrize("precision", [rsg.RSG_FP64, rsg.RSG_FP32])
+def test_members_with_different_tunings(precision):
+    """Each launch covers the members of one kernel configuration; results equal alone."""
+    jobs = mix()
+    solo = [alone(c, n, precision) for c, n in jobs]
+    b = rsg.Batch(precision, steps_per_block=11)
+    solvers = [rsg.DeviceSolver.from_case(c, precision, tuning=t) for (c, _), t in zip(jobs, TUNED)]
+    for s, (_, n) in zip(solvers, jobs):
+        b.add(s, n)
+    assert b.run(1000) == 0
+    b.synchronize()
+    for s, (c, n), ref in zip(solvers, jobs, solo):
+        V, S = s.get_state()
+        assert np.array_equal(V, ref[0]) and np.array_equal(S, ref[1]), c.name
+        assert np.array_equal(s.traces(), ref[3]), c.name
