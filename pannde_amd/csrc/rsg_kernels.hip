@@ -52,6 +52,17 @@
 #define RSG_VELOCITY_ATTR
 #endif
 
+// Loads of data read once per step (links, keys, gather lists, per-entity C): non-temporal
+// by default; RSG_CACHED_LINKS=1 makes them ordinary loads so they can stay cached.
+#ifndef RSG_CACHED_LINKS
+#define RSG_CACHED_LINKS 0
+#endif
+#if RSG_CACHED_LINKS
+#define STREAM_LOAD(p) (*(p))
+#else
+#define STREAM_LOAD(p) __builtin_nontemporal_load(p)
+#endif
+
 namespace rsg {
 namespace {
 
@@ -191,15 +202,15 @@ __device__ __forceinline__ void axis_spacings(const T* __restrict__ axis, uint32
 // 8 x int32 of one entity as two 16-byte loads; read once per step, so non-temporal.
 __device__ __forceinline__ void load_conn(const int32_t* __restrict__ conn, int i, int (&c)[8]) {
   const v4i* p = reinterpret_cast<const v4i*>(conn) + 2 * (size_t)i;
-  const v4i lo = __builtin_nontemporal_load(p);
-  const v4i hi = __builtin_nontemporal_load(p + 1);
+  const v4i lo = STREAM_LOAD(p);
+  const v4i hi = STREAM_LOAD(p + 1);
   c[0] = lo.x; c[1] = lo.y; c[2] = lo.z; c[3] = lo.w;
   c[4] = hi.x; c[5] = hi.y; c[6] = hi.z; c[7] = hi.w;
 }
 
 // 8 x uint16 block-local indices of one entity as one 16-byte load (0xFFFF = none).
 __device__ __forceinline__ void load_lconn(const uint16_t* __restrict__ lconn, int i, int (&c)[8]) {
-  const v4i w = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(lconn) + (size_t)i);
+  const v4i w = STREAM_LOAD(reinterpret_cast<const v4i*>(lconn) + (size_t)i);
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const uint32_t u = (uint32_t)w[k];
@@ -212,7 +223,7 @@ __device__ __forceinline__ void load_lconn(const uint16_t* __restrict__ lconn, i
 // 8 block-local indices (10 bits each, kNoLink = none) and the key of one entity, one
 // 16-byte load.
 __device__ __forceinline__ uint32_t load_lkey(const uint32_t* __restrict__ lkey, int i, int (&c)[8]) {
-  const v4i w = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(lkey) + (size_t)i);
+  const v4i w = STREAM_LOAD(reinterpret_cast<const v4i*>(lkey) + (size_t)i);
   const uint32_t w0 = (uint32_t)w.x, w1 = (uint32_t)w.y, w2 = (uint32_t)w.z;
   const uint32_t l[8] = {w0 & kNoLink, (w0 >> 10) & kNoLink, (w0 >> 20) & kNoLink, w1 & kNoLink,
                          (w1 >> 10) & kNoLink, (w1 >> 20) & kNoLink, w2 & kNoLink, (w2 >> 10) & kNoLink};
@@ -235,7 +246,7 @@ __device__ __forceinline__ void stage(T* lds, int max_u, const int32_t* __restri
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = threadIdx.x + u * blockDim.x;
-      g[u] = __builtin_nontemporal_load(ulist + u0 + (i < nu ? i : 0));
+      g[u] = STREAM_LOAD(ulist + u0 + (i < nu ? i : 0));
     }
     T v[U][NCOMP];
 #pragma unroll
@@ -251,7 +262,7 @@ __device__ __forceinline__ void stage(T* lds, int max_u, const int32_t* __restri
     }
   } else {
     for (int i = threadIdx.x; i < nu; i += blockDim.x) {
-      const int g = __builtin_nontemporal_load(ulist + u0 + i);
+      const int g = STREAM_LOAD(ulist + u0 + i);
 #pragma unroll
       for (int k = 0; k < NCOMP; k++) lds[k * max_u + i] = src[k][g];
     }
@@ -278,7 +289,7 @@ __device__ __forceinline__ void velocity_load(const VelocityArgs<T>& a, int n, V
     load_conn(a.ncell, n, in.c);
   }
   if (keyed<KM>(a)) {
-    if (!(LDS && packed<KM>(a))) in.key = __builtin_nontemporal_load(a.key + n);
+    if (!(LDS && packed<KM>(a))) in.key = STREAM_LOAD(a.key + n);
     in.idt = in.sx = in.sy = in.sz = T(0);
   } else {
     in.key = 0;
@@ -411,7 +422,7 @@ __device__ __forceinline__ T cload(const T* __restrict__ C, int64_t ldc, int j, 
   // ldc > 0: [21][ldc] arrays; ldc == 0: blocks of 64 cells, [n/64][21][64], so one
   // wavefront's 21 coefficients are a single contiguous run
   const int64_t idx = ldc ? (int64_t)j * ldc + i : ((int64_t)(i >> 6) * 21 + j) * 64 + (i & 63);
-  return __builtin_nontemporal_load(C + idx);
+  return STREAM_LOAD(C + idx);
 }
 
 // Index of C_(I,J) (0-based Voigt indices) in the 21-entry upper triangle.
@@ -441,7 +452,7 @@ __device__ __forceinline__ void stress_load(const StressArgs<T>& a, int e, StrIn
     load_conn(a.cnode, e, in.c);
   }
   if (keyed<KM>(a)) {
-    if (!(LDS && packed<KM>(a))) in.key = __builtin_nontemporal_load(a.key + e);
+    if (!(LDS && packed<KM>(a))) in.key = STREAM_LOAD(a.key + e);
     in.sx = in.sy = in.sz = T(0);
 #pragma unroll
     for (int j = 0; j < 21; j++) in.cc[j] = T(0);
