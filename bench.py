@@ -207,7 +207,9 @@ def main():
     layout = {"coefficients": ("class keys: %d cell materials, %d node materials" % (tun["cell_classes"],
                                                                                    tun["node_classes"])
                                if tun["cell_classes"] >= 0 else "per-entity arrays"),
-              "links": "16-bit block-local (LDS staging)" if tun["lds_stress"] > 0 else "32-bit global"}
+              "links": {2: "10-bit block-local packed with the key (LDS staging)",
+                        1: "16-bit block-local (LDS staging)", 0: "32-bit global"}[tun["links_stress"]],
+              "cells_per_thread_stress": tun["per_thread_stress"]}
 
     out = {
         "metric": "Gcell-steps/s FP64 vs FP32 @<=1e-4 trace err; served solves/hr + p50/p99 lat",
