@@ -13,7 +13,9 @@ timed steps; roofline = the stress kernel's algorithmic bytes per launch over it
 launch duration (HIP events on the solver's stream, same timed region); cpu_baseline =
 the reference solver itself (oracle/_ref/ref_rsg, built from the reference headers) under
 mpirun on this host's cores, on a bounded sample of the same job; fp32 = the mixed-
-precision form on the same job with its sensor-trace error against FP64; batched = 64
+precision form on the same job with its sensor-trace error against FP64; bracket = the
+cracked Ti-6Al-4V L-bracket (9.4 M cells, 4000 steps) in both forms with the FP32 trace error;
+batched = 64
 independent instances (32^3..128^3 cells, 1500 steps, 50-step graph blocks) advanced
 together in FP32, every instance's traces checked against the same batch run in FP64;
 served = a bounded sample (the first 120 requests) of the seeded inspection stream (cracked brackets and plates,
@@ -119,6 +121,28 @@ def batched(rsg, cases, seed, steps_per_block=50):
     return out
 
 
+def bracket(rsg, cases):
+    """The cracked Ti-6Al-4V L-bracket (2 x 512x96x96 cells, 4000 steps) in FP64 and FP32:
+    rates and the per-trace error of the mixed-precision form."""
+    from pannde_amd import serve
+
+    case = cases.ti_bracket(seed=5)
+    out = {"cells": case.n_cells, "nodes": case.n_nodes, "steps": case.nsteps,
+           "crack_duplicated_nodes": case.meta["n_dup"]}
+    tr = {}
+    for prec, tag in ((rsg.RSG_FP64, "fp64"), (rsg.RSG_FP32, "fp32")):
+        r = run_timed(rsg, case, prec, 0, case.nsteps)
+        tr[tag] = r
+        out[tag] = round(case.n_cells * case.nsteps / r["seconds"] / 1e9, 4)
+    err = serve.rel_l2_traces(tr["fp32"]["traces"], tr["fp64"]["traces"])
+    out["max_trace_rel_l2_fp32_vs_fp64"] = float("%.3e" % err.max())
+    out["wavefield_rel_l2_fp32_vs_fp64"] = float("%.3e" % max(rel_l2(tr["fp32"]["V"], tr["fp64"]["V"]),
+                                                              rel_l2(tr["fp32"]["S"], tr["fp64"]["S"])))
+    log("bench: bracket %d cells, %d steps: FP64 %.3f, FP32 %.3f Gcell-steps/s, trace err %.2e"
+        % (case.n_cells, case.nsteps, out["fp64"], out["fp32"], err.max()))
+    return out
+
+
 def served(rsg, n, rate, seed):
     from pannde_amd import serve
 
@@ -167,6 +191,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-bracket", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-seed", type=int, default=4)
     ap.add_argument("--serve-requests", type=int, default=120)
@@ -254,6 +279,10 @@ def main():
                        "stress_kernel_gbs": round(STRESS_BYTES["f32"] * nc / s32 / 1e9, 1)}
         log("bench: FP32 %.3f Gcell-steps/s (x%.2f), trace err %.2e, wavefield err %.2e"
             % (v32, v32 / value, max(tr_err), wf_err))
+
+    if not args.no_bracket:
+        log("bench: cracked L-bracket, FP64 and FP32")
+        out["bracket"] = bracket(rsg, cases)
 
     if not args.no_batched:
         log("bench: batched FP32 instances (seed %d)" % args.batch_seed)
