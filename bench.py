@@ -76,11 +76,11 @@ def run_timed(rsg, case, precision, warmup, steps):
                 dev_bytes=dev_bytes, tuning=tuning)
 
 
-def create_all(rsg, case_list, precision, threads=8):
+def create_all(rsg, case_list, precision, threads=8, tuning=None):
     import concurrent.futures as cf
 
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-        return list(ex.map(lambda c: rsg.DeviceSolver.from_case(c, precision), case_list))
+        return list(ex.map(lambda c: rsg.DeviceSolver.from_case(c, precision, tuning=tuning), case_list))
 
 
 def batched(rsg, cases, seed, steps_per_block=50):
@@ -95,7 +95,7 @@ def batched(rsg, cases, seed, steps_per_block=50):
     traces = {}
     for prec, tag in ((rsg.RSG_FP32, "fp32"), (rsg.RSG_FP64, "fp64")):
         t0 = time.perf_counter()
-        solvers = create_all(rsg, inst, prec)
+        solvers = create_all(rsg, inst, prec, tuning=serve.BATCH_TUNING)
         t_create = time.perf_counter() - t0
         b = rsg.Batch(prec, steps_per_block=steps_per_block)
         for s in solvers:

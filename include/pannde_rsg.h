@@ -209,10 +209,14 @@ int rsg_last_error(char* buf, size_t len);
  * every instance of a group (a device work table of (instance, block) items and a device
  * table of per-instance arguments), and the K consecutive steps of a block are captured
  * once in a HIP graph.  Instances are grouped so that a group's state arrays (V and S)
- * fit a byte budget (by default 256 MiB in FP32 and 192 MiB in FP64, sized to the MI355X's
- * 256 MiB last-level cache);
- * each group takes all K steps of a block before the next group starts, so the state one
- * kernel writes is still cached when the other reads it.  Instances join and leave between K-step blocks; each
+ * fit a byte budget (by default 128 MiB in FP32 and 96 MiB in FP64).  Each instance joins
+ * one of two lanes, the one with the fewest state bytes still to step; lanes are
+ * independent schedulers on streams of their own that run at once, so the two groups in
+ * flight fit the MI355X's 256 MiB last-level cache and each lane's launches fill the
+ * tails of the other's.  In a lane, each group takes all K steps of a block before the
+ * next group starts, so the state one kernel writes is still cached when the other reads
+ * it.
+ * Instances join and leave between K-step blocks; each
  * takes exactly its own number of steps.  No counterpart in the reference, which runs
  * one model per process; each instance's arithmetic is that of rsg_step, so its results
  * are bit-identical alone and in any mix.  Instances are ordinary handles of the batch's
@@ -235,7 +239,7 @@ int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot);
 
 /* Detaches a slot's handle (finished or not); the handle can be stepped alone again.
  * A finished instance waits only for the block in which it finished, so later blocks of
- * the batch can run meanwhile; an unfinished one waits for the batch's stream. */
+ * the batch can run meanwhile; an unfinished one waits for its lane. */
 int rsg_batch_remove(rsg_batch b, int32_t slot);
 
 /* Runs up to nblocks K-step blocks (enqueued; see rsg_batch_synchronize).  *unfinished
@@ -243,13 +247,17 @@ int rsg_batch_remove(rsg_batch b, int32_t slot);
 int rsg_batch_run(rsg_batch b, int32_t nblocks, int32_t* unfinished);
 
 /* Scheduling options (results do not depend on them):
- *   RSG_BATCH_GROUP_BYTES  state bytes per group (>= 0; 0: all instances in one group;
- *                          default 256 MiB in FP32, 192 MiB in FP64)
+ *   RSG_BATCH_GROUP_BYTES  state bytes per group (>= 0; 0: all instances of a lane in one
+ *                          group; default 128 MiB in FP32, 96 MiB in FP64)
  *   RSG_BATCH_XCD_RUN      work-item order across the 8 XCDs: -1 table order, 0 one
  *                          contiguous eighth per XCD, n > 0 runs of n items per XCD in turn
- *                          (default 64) */
+ *                          (default 64)
+ *   RSG_BATCH_LANES        instances added from now on join the lane with the fewest
+ *                          state bytes still to step among this many (1..4, default 2);
+ *                          members keep their lane */
 #define RSG_BATCH_GROUP_BYTES 1
 #define RSG_BATCH_XCD_RUN 2
+#define RSG_BATCH_LANES 3
 int rsg_batch_set_option(rsg_batch b, int32_t option, int64_t value);
 
 int rsg_batch_synchronize(rsg_batch b);

@@ -1540,64 +1540,91 @@ struct rsg_batch_s {
   bool dense = false;   // one workgroup per work item
   bool use_graph = true;
   // instances are scheduled in groups whose state arrays (V and S) fit this many bytes;
-  // each group takes all K steps of a block before the next starts (0: one group).  Set at
-  // creation: 256 MiB for FP32, 192 MiB for FP64 (measured optima on 64 instances of
-  // 32^3..128^3 cells)
-  int64_t group_bytes = (int64_t)192 << 20;
+  // each group takes all K steps of a block before the next group of its lane starts
+  // (0: one group per lane).  Set at creation: 128 MiB for FP32, 96 MiB for FP64 (measured
+  // optima with two lanes on 64 instances of 32^3..128^3 cells)
+  int64_t group_bytes = (int64_t)96 << 20;
   // work-table order across XCDs: -1 identity, 0 one contiguous eighth per XCD, n > 0 runs
   // of n consecutive items per XCD in turn
   int32_t xcd_run = 64;
-  hipStream_t stream = nullptr;
+  // an instance joins the lane with the fewest state bytes among the first `lanes`; lanes
+  // are independent schedulers, each on a stream of its own, so one lane's launches fill
+  // the tails of the other's and neither waits for the other at block ends
+  static constexpr int kMaxLanes = 4;
+  int32_t lanes = 2;
   struct Slot {
     rsg_instance* h = nullptr;
     int64_t total = 0, done = 0;
+    int32_t lane = 0;
     bool attached = false;
     hipEvent_t fin = nullptr;   // recorded after the block in which the instance finishes
     bool fin_recorded = false;
   };
   std::vector<Slot> slots;
-  // control block on the device: descriptors | velocity tables | stress tables | amplitudes.
-  // Each group's table is {n, flags, (instance, block) x n}, 256-byte aligned, with room
-  // for n rounded up to kItemQuantum so small changes of membership keep the layout.
-  size_t cap_inst = 0, cap_wv = 0, cap_ws = 0, cap_amp = 0;  // cap_w*: int32 words
-  size_t off_wv = 0, off_ws = 0, off_amp = 0, ctl_bytes = 0;
-  char* ctl_dev = nullptr;
-  char* ctl_host[2] = {nullptr, nullptr};
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  bool busy[2] = {false, false};
-  int cur = 0;
-  // launch layout of the current block: per group, table word offsets and grids
+  // launch layout of a lane's block: per group, table word offsets, lengths and grids
   struct Launch {
     int64_t off_v, off_s;
+    int32_t n_v, n_s;          // table lengths (padded)
     int32_t grid_v, grid_s;
     int32_t km_v, km_s;        // coefficient forms (0 arrays, 1 keys, 3 keys packed with links)
     bool lds_v, lds_s;         // LDS staging
     int32_t max_u_v, max_u_s;  // largest staged set of a member
     int32_t block_v, block_s, cpt_v, cpt_s;  // the members' workgroup sizes and per-thread counts
     bool operator==(const Launch& o) const {
-      return off_v == o.off_v && off_s == o.off_s && grid_v == o.grid_v && grid_s == o.grid_s &&
-             km_v == o.km_v && km_s == o.km_s && lds_v == o.lds_v && lds_s == o.lds_s &&
+      return off_v == o.off_v && off_s == o.off_s && n_v == o.n_v && n_s == o.n_s && grid_v == o.grid_v &&
+             grid_s == o.grid_s && km_v == o.km_v && km_s == o.km_s && lds_v == o.lds_v && lds_s == o.lds_s &&
              max_u_v == o.max_u_v && max_u_s == o.max_u_s && block_v == o.block_v && block_s == o.block_s &&
              cpt_v == o.cpt_v && cpt_s == o.cpt_s;
     }
   };
-  std::vector<Launch> layout, graph_layout;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  bool graph_dirty = true;
+  struct Lane {
+    hipStream_t stream = nullptr;
+    // control block on the device: descriptors (one per slot) | velocity tables | stress
+    // tables | amplitudes.  Each group's table holds its (instance, block) items in launch
+    // order (XCD order applied), padded with (-1, 0) to a multiple of kItemQuantum items so
+    // small changes of membership keep the layout; tables are 256-byte aligned.  The host
+    // copy is double-buffered so the next block is prepared while the last one uploads.
+    size_t cap_inst = 0, cap_wv = 0, cap_ws = 0, cap_amp = 0;  // cap_w*: int32 words
+    size_t off_wv = 0, off_ws = 0, off_amp = 0, ctl_bytes = 0;
+    char* ctl_dev = nullptr;
+    char* ctl_host[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool busy[2] = {false, false};
+    int cur = 0;
+    std::vector<Launch> layout, graph_layout;
+    hipGraph_t graph = nullptr;  // the lane's K-step block, replayed while its layout holds
+    hipGraphExec_t exec = nullptr;
+
+    void drop_graph() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+      graph_layout.clear();
+    }
+    void release() {
+      if (stream) (void)hipStreamSynchronize(stream);
+      drop_graph();
+      if (ctl_dev) (void)hipFree(ctl_dev);
+      ctl_dev = nullptr;
+      for (int i = 0; i < 2; i++) {
+        if (ctl_host[i]) (void)hipHostFree(ctl_host[i]);
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+        ctl_host[i] = nullptr;
+        ev[i] = nullptr;
+      }
+      if (stream) (void)hipStreamDestroy(stream);
+      stream = nullptr;
+    }
+  };
+  Lane lane[kMaxLanes];
 
   size_t desc_size() const {
     return precision == RSG_FP64 ? sizeof(rsg::BatchDesc<double>) : sizeof(rsg::BatchDesc<float>);
   }
-  void drop_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
-    graph_dirty = true;
-  }
   ~rsg_batch_s() {
-    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& ln : lane)
+      if (ln.stream) (void)hipStreamSynchronize(ln.stream);
     for (auto& sl : slots) {
       if (sl.attached && sl.h) {
         sl.h->in_batch = false;
@@ -1605,13 +1632,7 @@ struct rsg_batch_s {
       }
       if (sl.fin) (void)hipEventDestroy(sl.fin);
     }
-    drop_graph();
-    if (ctl_dev) (void)hipFree(ctl_dev);
-    for (int i = 0; i < 2; i++) {
-      if (ctl_host[i]) (void)hipHostFree(ctl_host[i]);
-      if (ev[i]) (void)hipEventDestroy(ev[i]);
-    }
-    if (stream) (void)hipStreamDestroy(stream);
+    for (auto& ln : lane) ln.release();
   }
 };
 
@@ -1620,78 +1641,99 @@ namespace {
 constexpr int32_t kItemQuantum = 256;
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
-size_t table_words(size_t n) {  // one group's table: header + items, 64-word aligned
-  const size_t q = (n + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
-  return (2 + 2 * q + 63) / 64 * 64;
+size_t padded_items(size_t n) { return (n + kItemQuantum - 1) / kItemQuantum * kItemQuantum; }
+size_t table_words(size_t n) { return (2 * padded_items(n) + 63) / 64 * 64; }  // 64-word aligned
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup p -> XCD p % 8).  The
+// item launched at position p of an n-item table: run < 0 keeps table order; run == 0 gives
+// XCD x the x-th contiguous eighth of the table (the first n % 8 eighths one item longer);
+// run > 0 gives each XCD runs of `run` consecutive items in turn (run r of XCD x is items
+// [(8r + x) run, (8r + x + 1) run)), with the last n mod 8 run items in place.  Each
+// mapping is a permutation of [0, n).
+int32_t launch_order(int32_t p, int32_t n, int32_t run) {
+  if (run < 0) return p;
+  const int32_t x = p & 7, j = p >> 3;
+  if (run == 0) {
+    const int32_t q = n >> 3, r = n & 7;
+    return x * q + std::min(x, r) + j;
+  }
+  const int32_t n8 = n - n % (8 * run);
+  if (p >= n8) return p;
+  const int32_t g = j / run;
+  return (g * 8 + x) * run + (j - g * run);
 }
 
 int64_t state_bytes(const rsg_batch_s* b, const rsg_instance* h) {
   return (6 * (int64_t)h->nc + 3 * (int64_t)h->nn) * (b->precision == RSG_FP64 ? 8 : 4);
 }
 
-// Grows the control block to hold the current instances (amplitude area for K steps).
-int batch_reserve(rsg_batch_s* b) {
-  size_t ninst = b->slots.size(), nwv = 0, nws = 0, namp = 0;
+bool active(const rsg_batch_s::Slot& sl) { return sl.attached && sl.done < sl.total; }
+
+// Grows a lane's control block to hold its instances (amplitude area for K steps).
+int batch_reserve(rsg_batch_s* b, int l) {
+  rsg_batch_s::Lane& ln = b->lane[l];
+  size_t ninst = b->slots.size(), nwv = 0, nws = 0, namp = 0, nmem = 0;
   for (auto& sl : b->slots) {
-    if (!sl.attached) continue;
+    if (!sl.attached || sl.lane != l) continue;
     rsg_instance* h = sl.h;
     nwv += (size_t)(h->nn + h->block_v * h->cpt_v - 1) / (h->block_v * h->cpt_v);
     nws += (size_t)(h->nc + h->block_s * h->cpt_s - 1) / (h->block_s * h->cpt_s);
     namp += std::max<size_t>(1, h->channels.size()) * (size_t)b->K;
+    nmem++;
   }
-  // every group table wastes at most 2 + 2 * (kItemQuantum - 1) + 63 words; there are at
-  // most as many groups as instances
+  // every group table wastes at most 2 * (kItemQuantum - 1) + 63 words; there are at most
+  // as many groups as members
   const size_t slack = (size_t)(2 + 2 * kItemQuantum + 64);
-  const size_t wv = 2 * nwv + slack * std::max<size_t>(1, ninst), ws = 2 * nws + slack * std::max<size_t>(1, ninst);
-  if (ninst <= b->cap_inst && wv <= b->cap_wv && ws <= b->cap_ws && namp <= b->cap_amp && b->ctl_dev)
+  const size_t wv = 2 * nwv + slack * std::max<size_t>(1, nmem), ws = 2 * nws + slack * std::max<size_t>(1, nmem);
+  if (ninst <= ln.cap_inst && wv <= ln.cap_wv && ws <= ln.cap_ws && namp <= ln.cap_amp && ln.ctl_dev)
     return RSG_OK;
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  b->cap_inst = std::max(b->cap_inst, ninst + ninst / 2 + 4);
-  b->cap_wv = std::max(b->cap_wv, wv + wv / 2 + 4 * slack);
-  b->cap_ws = std::max(b->cap_ws, ws + ws / 2 + 4 * slack);
-  b->cap_amp = std::max(b->cap_amp, namp + namp / 2 + 64);
-  b->off_wv = align256(b->cap_inst * b->desc_size());
-  b->off_ws = b->off_wv + align256(b->cap_wv * sizeof(int32_t));
-  b->off_amp = b->off_ws + align256(b->cap_ws * sizeof(int32_t));
-  b->ctl_bytes = b->off_amp + align256(b->cap_amp * sizeof(double));
-  if (b->ctl_dev) (void)hipFree(b->ctl_dev);
-  b->ctl_dev = nullptr;
-  HIP_TRY(hipMalloc((void**)&b->ctl_dev, b->ctl_bytes));
+  HIP_TRY(hipStreamSynchronize(ln.stream));
+  ln.cap_inst = std::max(ln.cap_inst, ninst + ninst / 2 + 4);
+  ln.cap_wv = std::max(ln.cap_wv, wv + wv / 2 + 4 * slack);
+  ln.cap_ws = std::max(ln.cap_ws, ws + ws / 2 + 4 * slack);
+  ln.cap_amp = std::max(ln.cap_amp, namp + namp / 2 + 64);
+  ln.off_wv = align256(ln.cap_inst * b->desc_size());
+  ln.off_ws = ln.off_wv + align256(ln.cap_wv * sizeof(int32_t));
+  ln.off_amp = ln.off_ws + align256(ln.cap_ws * sizeof(int32_t));
+  ln.ctl_bytes = ln.off_amp + align256(ln.cap_amp * sizeof(double));
+  if (ln.ctl_dev) (void)hipFree(ln.ctl_dev);
+  ln.ctl_dev = nullptr;
+  HIP_TRY(hipMalloc((void**)&ln.ctl_dev, ln.ctl_bytes));
   for (int i = 0; i < 2; i++) {
-    if (b->ctl_host[i]) (void)hipHostFree(b->ctl_host[i]);
-    b->ctl_host[i] = nullptr;
-    HIP_TRY(hipHostMalloc((void**)&b->ctl_host[i], b->ctl_bytes, hipHostMallocDefault));
-    b->busy[i] = false;
+    if (ln.ctl_host[i]) (void)hipHostFree(ln.ctl_host[i]);
+    ln.ctl_host[i] = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&ln.ctl_host[i], ln.ctl_bytes, hipHostMallocDefault));
+    ln.busy[i] = false;
   }
-  b->drop_graph();  // the graph's kernels point into the control block
+  ln.drop_graph();  // the graph's kernels point into the control block
   return RSG_OK;
 }
 
-// Enqueues one K-step block: every group in turn takes its K steps (velocity then stress
-// per step).  The launches form a chain of kernel nodes in a graph built explicitly (not
-// by stream capture, so other threads' HIP calls cannot disturb it) and replayed while
-// the launch layout stays the same.
+// Enqueues one K-step block of a lane: every group in turn takes its K steps (velocity
+// then stress per step).  The launches form a chain of kernel nodes in a graph built
+// explicitly (not by stream capture, so other threads' HIP calls cannot disturb it) and
+// replayed while the lane's launch layout stays the same.
 template <typename T>
-int batch_launch_block(rsg_batch_s* b) {
-  const auto* desc = (const rsg::BatchDesc<T>*)b->ctl_dev;
-  const int32_t* wv = (const int32_t*)(b->ctl_dev + b->off_wv);
-  const int32_t* ws = (const int32_t*)(b->ctl_dev + b->off_ws);
+int batch_launch_block(rsg_batch_s* b, rsg_batch_s::Lane& ln) {
+  const auto* desc = (const rsg::BatchDesc<T>*)ln.ctl_dev;
+  const int32_t* wv = (const int32_t*)(ln.ctl_dev + ln.off_wv);
+  const int32_t* ws = (const int32_t*)(ln.ctl_dev + ln.off_ws);
   auto emit = [&](hipGraph_t graph, hipGraphNode_t* node) -> hipError_t {
-    for (const auto& g : b->layout)
+    for (const auto& g : ln.layout)
       for (int k = 0; k < b->K; k++) {
         hipError_t e;
         if (sizeof(T) == 8) {
-          e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv + g.off_v, k, g.grid_v, g.block_v,
-                                    g.lds_v, g.max_u_v, g.cpt_v, g.km_v, true, b->stream, graph, node);
+          e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, wv + g.off_v, g.n_v, k, g.grid_v, g.block_v,
+                                    g.lds_v, g.max_u_v, g.cpt_v, g.km_v, true, ln.stream, graph, node);
           if (e != hipSuccess) return e;
-          e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws + g.off_s, k, g.grid_s, g.block_s,
-                                    g.lds_s, g.max_u_s, g.cpt_s, g.km_s, false, b->stream, graph, node);
+          e = rsg::launch_batch_f64((const rsg::BatchDesc<double>*)desc, ws + g.off_s, g.n_s, k, g.grid_s, g.block_s,
+                                    g.lds_s, g.max_u_s, g.cpt_s, g.km_s, false, ln.stream, graph, node);
         } else {
-          e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv + g.off_v, k, g.grid_v, g.block_v,
-                                    g.lds_v, g.max_u_v, g.cpt_v, g.km_v, true, b->stream, graph, node);
+          e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, wv + g.off_v, g.n_v, k, g.grid_v, g.block_v,
+                                    g.lds_v, g.max_u_v, g.cpt_v, g.km_v, true, ln.stream, graph, node);
           if (e != hipSuccess) return e;
-          e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws + g.off_s, k, g.grid_s, g.block_s,
-                                    g.lds_s, g.max_u_s, g.cpt_s, g.km_s, false, b->stream, graph, node);
+          e = rsg::launch_batch_f32((const rsg::BatchDesc<float>*)desc, ws + g.off_s, g.n_s, k, g.grid_s, g.block_s,
+                                    g.lds_s, g.max_u_s, g.cpt_s, g.km_s, false, ln.stream, graph, node);
         }
         if (e != hipSuccess) return e;
       }
@@ -1702,54 +1744,53 @@ int batch_launch_block(rsg_batch_s* b) {
     if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "batch launch: %s", hipGetErrorString(e));
     return RSG_OK;
   }
-  if (b->graph_dirty || !b->exec || b->graph_layout != b->layout) {
-    b->drop_graph();
-    HIP_TRY(hipGraphCreate(&b->graph, 0));
+  if (!ln.exec || ln.graph_layout != ln.layout) {
+    ln.drop_graph();
+    HIP_TRY(hipGraphCreate(&ln.graph, 0));
     hipGraphNode_t node = nullptr;
-    hipError_t e = emit(b->graph, &node);
+    hipError_t e = emit(ln.graph, &node);
     if (e != hipSuccess) {
-      b->drop_graph();
+      ln.drop_graph();
       return fail(RSG_ERR_DEVICE, "batch graph: %s", hipGetErrorString(e));
     }
-    HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    b->graph_layout = b->layout;
-    b->graph_dirty = false;
+    HIP_TRY(hipGraphInstantiate(&ln.exec, ln.graph, nullptr, nullptr, 0));
+    ln.graph_layout = ln.layout;
   }
-  HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  HIP_TRY(hipGraphLaunch(ln.exec, ln.stream));
   return RSG_OK;
 }
 
-// Fills one K-step block's control data (descriptors, group work tables, amplitudes) for
-// the instances with steps left, uploads it and launches the block.
+// Fills one K-step block's control data of lane l (descriptors, group work tables,
+// amplitudes) for its instances with steps left, uploads it and launches the block.
 template <typename T>
-int batch_block(rsg_batch_s* b, int32_t* unfinished) {
+int batch_block(rsg_batch_s* b, int l, int32_t* unfinished) {
   int rc;
-  if ((rc = batch_reserve(b))) return rc;
-  const int j = b->cur;
-  if (b->busy[j]) HIP_TRY(hipEventSynchronize(b->ev[j]));
-  char* host = b->ctl_host[j];
+  if ((rc = batch_reserve(b, l))) return rc;
+  rsg_batch_s::Lane& ln = b->lane[l];
+  const int j = ln.cur;
+  if (ln.busy[j]) HIP_TRY(hipEventSynchronize(ln.ev[j]));
+  char* host = ln.ctl_host[j];
   auto* desc = (rsg::BatchDesc<T>*)host;
-  int32_t* wv = (int32_t*)(host + b->off_wv);
-  int32_t* ws = (int32_t*)(host + b->off_ws);
-  double* amp = (double*)(host + b->off_amp);
+  int32_t* wv = (int32_t*)(host + ln.off_wv);
+  int32_t* ws = (int32_t*)(host + ln.off_ws);
+  double* amp = (double*)(host + ln.off_amp);
   size_t amp_off = 0;
+  auto mine = [&](const rsg_batch_s::Slot& sl) { return active(sl) && sl.lane == l; };
   // traces that would overflow their device buffer are flushed first (needs the stream idle)
   bool flush = false;
   for (auto& sl : b->slots)
-    if (sl.attached && sl.done < sl.total && !sl.h->sensors.empty() &&
-        sl.h->trace_dev_steps + b->K > kTraceCapSteps)
-      flush = true;
+    if (mine(sl) && !sl.h->sensors.empty() && sl.h->trace_dev_steps + b->K > kTraceCapSteps) flush = true;
   if (flush) {
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(ln.stream));
     for (auto& sl : b->slots)
-      if (sl.attached && (rc = flush_traces(sl.h))) return rc;
+      if (sl.attached && sl.lane == l && (rc = flush_traces(sl.h))) return rc;
   }
   std::vector<int> kmax(b->slots.size(), 0);
   for (size_t i = 0; i < b->slots.size(); i++) {
     auto& sl = b->slots[i];
     rsg::BatchDesc<T>& d = desc[i];
     memset(&d, 0, sizeof(d));
-    if (!sl.attached || sl.done >= sl.total) continue;
+    if (!mine(sl)) continue;
     rsg_instance* h = sl.h;
     if (h->tables_dirty && (rc = build_tables(h))) return rc;
     const int km = (int)std::min<int64_t>(b->K, sl.total - sl.done);
@@ -1761,7 +1802,7 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
     d.s = sa;
     const size_t nch = std::max<size_t>(1, h->channels.size());
     d.nch = (int32_t)nch;
-    d.amp_base = (const double*)(b->ctl_dev + b->off_amp) + amp_off;
+    d.amp_base = (const double*)(ln.ctl_dev + ln.off_amp) + amp_off;
     d.row = (int32_t)(h->sensors.size() * 3);
     d.trace_base = d.row ? h->trace.p : nullptr;
     d.trace_row0 = h->trace_dev_steps;
@@ -1782,7 +1823,6 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
   // per-thread count of each kernel, so each launch runs one kernel variant), in slot order
   // within a configuration, each group within the state budget (an instance larger than the
   // budget forms a group of its own)
-  const int32_t flags = b->xcd_run < 0 ? 0 : 1 | (b->xcd_run << 8);
   auto config = [](const rsg_instance* h) -> std::array<int32_t, 6> {
     return {h->v_lkey ? 3 : h->nkey ? 1 : 0, h->s_lkey ? 3 : h->ckey ? 1 : 0, h->block_v, h->block_s, h->cpt_v,
             h->cpt_s};
@@ -1793,9 +1833,10 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
       const auto f = config(b->slots[i].h);
       if (std::find(configs.begin(), configs.end(), f) == configs.end()) configs.push_back(f);
     }
-  b->layout.clear();
+  ln.layout.clear();
   size_t ov = 0, os = 0;
   std::vector<size_t> members;
+  std::vector<int32_t> natural;  // a table's items in instance order, before launch ordering
   for (const auto& f : configs) {
     const int32_t cv = f[2] * f[4], cs = f[3] * f[5];
     for (size_t i = 0; i < b->slots.size();) {
@@ -1814,15 +1855,9 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
         ns += (h->nc + cs - 1) / cs;
       }
       if (members.empty()) break;
-      if (ov + table_words(nv) > b->cap_wv || os + table_words(ns) > b->cap_ws)
+      if (ov + table_words(nv) > ln.cap_wv || os + table_words(ns) > ln.cap_ws)
         return fail(RSG_ERR_STATE, "batch work tables overflow");
-      int32_t* tv = wv + ov;
-      int32_t* ts = ws + os;
-      tv[0] = nv;
-      tv[1] = flags;
-      ts[0] = ns;
-      ts[1] = flags;
-      int32_t iv = 0, is = 0;
+      const int32_t qv = (int32_t)padded_items(nv), qs = (int32_t)padded_items(ns);
       // LDS staging for a kernel when every member has a plan, sized for the largest set
       bool lds_v = true, lds_s = true;
       int32_t mu_v = 0, mu_s = 0;
@@ -1832,21 +1867,31 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
         lds_s = lds_s && (h->s_lconn || h->s_lkey);
         mu_v = std::max(mu_v, h->v_maxu);
         mu_s = std::max(mu_s, h->s_maxu);
-        const int32_t nbv = (h->nn + cv - 1) / cv, nbs = (h->nc + cs - 1) / cs;
-        for (int32_t q = 0; q < nbv; q++, iv++) {
-          tv[2 + 2 * iv] = (int32_t)e;
-          tv[3 + 2 * iv] = q;
+      }
+      // items (instance, block) in slot order, then placed at their launch positions
+      for (int pass = 0; pass < 2; pass++) {
+        natural.clear();
+        for (size_t e : members) {
+          const rsg_instance* h = b->slots[e].h;
+          const int32_t nb = pass == 0 ? (h->nn + cv - 1) / cv : (h->nc + cs - 1) / cs;
+          for (int32_t q = 0; q < nb; q++) {
+            natural.push_back((int32_t)e);
+            natural.push_back(q);
+          }
         }
-        for (int32_t q = 0; q < nbs; q++, is++) {
-          ts[2 + 2 * is] = (int32_t)e;
-          ts[3 + 2 * is] = q;
+        const int32_t cnt = (int32_t)(natural.size() / 2), len = pass == 0 ? qv : qs;
+        int32_t* t = pass == 0 ? wv + ov : ws + os;
+        for (int32_t p = 0; p < len; p++) {
+          const int32_t src = launch_order(p, len, b->xcd_run);
+          t[2 * p] = src < cnt ? natural[2 * src] : -1;
+          t[2 * p + 1] = src < cnt ? natural[2 * src + 1] : 0;
         }
       }
-      const int32_t qv = (nv + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
-      const int32_t qs = (ns + kItemQuantum - 1) / kItemQuantum * kItemQuantum;
       rsg_batch_s::Launch g;
       g.off_v = (int64_t)ov;
       g.off_s = (int64_t)os;
+      g.n_v = qv;
+      g.n_s = qs;
       g.grid_v = b->dense ? qv : b->grid;
       g.grid_s = b->dense ? qs : b->grid;
       g.km_v = f[0];
@@ -1859,24 +1904,24 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
       g.lds_s = lds_s;
       g.max_u_v = lds_v ? mu_v : 0;
       g.max_u_s = lds_s ? mu_s : 0;
-      b->layout.push_back(g);
+      ln.layout.push_back(g);
       ov += table_words(nv);
       os += table_words(ns);
     }
   }
-  if (b->layout.empty()) {
+  if (ln.layout.empty()) {
     *unfinished = 0;
     return RSG_OK;
   }
-  HIP_TRY(hipMemcpyAsync(b->ctl_dev, host, b->ctl_bytes, hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(hipEventRecord(b->ev[j], b->stream));
-  b->busy[j] = true;
-  b->cur ^= 1;
-  if ((rc = batch_launch_block<T>(b))) return rc;
+  HIP_TRY(hipMemcpyAsync(ln.ctl_dev, host, ln.ctl_bytes, hipMemcpyHostToDevice, ln.stream));
+  HIP_TRY(hipEventRecord(ln.ev[j], ln.stream));
+  ln.busy[j] = true;
+  ln.cur ^= 1;
+  if ((rc = batch_launch_block<T>(b, ln))) return rc;
   int32_t left = 0;
   for (size_t i = 0; i < b->slots.size(); i++) {
     auto& sl = b->slots[i];
-    if (!sl.attached || kmax[i] == 0) continue;
+    if (kmax[i] == 0) continue;
     rsg_instance* h = sl.h;
     for (int k = 0; k < kmax[i]; k++) h->time += h->dt;
     h->steps += kmax[i];
@@ -1885,7 +1930,7 @@ int batch_block(rsg_batch_s* b, int32_t* unfinished) {
     if (sl.done < sl.total) {
       left++;
     } else if (!sl.fin_recorded) {  // later blocks leave this instance alone
-      HIP_TRY(hipEventRecord(sl.fin, b->stream));
+      HIP_TRY(hipEventRecord(sl.fin, ln.stream));
       sl.fin_recorded = true;
     }
   }
@@ -1907,15 +1952,18 @@ int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, i
   std::unique_ptr<rsg_batch_s> b(new (std::nothrow) rsg_batch_s());
   if (!b) return fail(RSG_ERR_NOMEM, "host allocation failed");
   b->precision = precision;
-  b->group_bytes = (int64_t)(precision == RSG_FP32 ? 256 : 192) << 20;
+  b->group_bytes = (int64_t)(precision == RSG_FP32 ? 128 : 96) << 20;
   b->K = steps_per_block ? steps_per_block : 50;
   b->use_graph = use_graph != 0;
   if (grid > 0)
     b->grid = grid;
   else
     b->dense = true;
-  HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  for (int i = 0; i < 2; i++) HIP_TRY(hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming));
+  // the lanes' streams are created together so that they land on different hardware queues
+  for (auto& ln : b->lane) {
+    HIP_TRY(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+    for (int i = 0; i < 2; i++) HIP_TRY(hipEventCreateWithFlags(&ln.ev[i], hipEventDisableTiming));
+  }
   *out = b.release();
   return RSG_OK;
 }
@@ -1941,15 +1989,23 @@ int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot) {
     id = (int32_t)b->slots.size();
     b->slots.emplace_back();
   }
+  // the lane with the fewest state bytes still to step
+  int64_t load[rsg_batch_s::kMaxLanes] = {};
+  for (auto& o : b->slots)
+    if (active(o)) load[o.lane] += state_bytes(b, o.h);
+  int32_t lane = 0;
+  for (int32_t l = 1; l < b->lanes; l++)
+    if (load[l] < load[lane]) lane = l;
   auto& sl = b->slots[id];
   if (!sl.fin) HIP_TRY(hipEventCreateWithFlags(&sl.fin, hipEventDisableTiming));
   sl.h = h;
   sl.total = nsteps;
   sl.done = 0;
+  sl.lane = lane;
   sl.attached = true;
   sl.fin_recorded = false;
   h->in_batch = true;
-  h->batch_stream = b->stream;
+  h->batch_stream = b->lane[lane].stream;
   *slot = id;
   return RSG_OK;
 }
@@ -1958,11 +2014,12 @@ int rsg_batch_remove(rsg_batch b, int32_t slot) {
   if (!b || slot < 0 || slot >= (int32_t)b->slots.size() || !b->slots[slot].attached)
     return fail(RSG_ERR_ARG, "no such slot %d", slot);
   auto& sl = b->slots[slot];
-  // a finished instance waits only for the block in which it finished
+  // a finished instance waits only for the block in which it finished, so later blocks of
+  // the batch can run meanwhile; an unfinished one waits for its lane
   if (sl.fin_recorded)
     HIP_TRY(hipEventSynchronize(sl.fin));
   else
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(b->lane[sl.lane].stream));
   sl.h->in_batch = false;
   sl.h->batch_stream = nullptr;
   sl.h = nullptr;
@@ -1976,10 +2033,20 @@ int rsg_batch_run(rsg_batch b, int32_t nblocks, int32_t* unfinished) {
   if (!b) return fail(RSG_ERR_ARG, "batch is null");
   int32_t left = 0;
   for (auto& sl : b->slots)
-    if (sl.attached && sl.done < sl.total) left++;
+    if (active(sl)) left++;
+  // one block per lane in turn, so the lanes advance together
   for (int32_t q = 0; q < nblocks && left > 0; q++) {
-    const int rc = b->precision == RSG_FP64 ? batch_block<double>(b, &left) : batch_block<float>(b, &left);
-    if (rc) return rc;
+    left = 0;
+    for (int l = 0; l < rsg_batch_s::kMaxLanes; l++) {
+      bool any = false;
+      for (auto& sl : b->slots) any = any || (active(sl) && sl.lane == l);
+      if (!any) continue;
+      int32_t lane_left = 0;
+      const int rc =
+          b->precision == RSG_FP64 ? batch_block<double>(b, l, &lane_left) : batch_block<float>(b, l, &lane_left);
+      if (rc) return rc;
+      left += lane_left;
+    }
   }
   if (unfinished) *unfinished = left;
   return RSG_OK;
@@ -1996,6 +2063,11 @@ int rsg_batch_set_option(rsg_batch b, int32_t option, int64_t value) {
       if (value < -1 || value > (1 << 20)) return fail(RSG_ERR_ARG, "xcd run %lld", (long long)value);
       b->xcd_run = (int32_t)value;
       return RSG_OK;
+    case RSG_BATCH_LANES:
+      if (value < 1 || value > rsg_batch_s::kMaxLanes)
+        return fail(RSG_ERR_ARG, "lanes %lld not in [1, %d]", (long long)value, rsg_batch_s::kMaxLanes);
+      b->lanes = (int32_t)value;
+      return RSG_OK;
     default:
       return fail(RSG_ERR_ARG, "unknown batch option %d", option);
   }
@@ -2003,7 +2075,7 @@ int rsg_batch_set_option(rsg_batch b, int32_t option, int64_t value) {
 
 int rsg_batch_synchronize(rsg_batch b) {
   if (!b) return fail(RSG_ERR_ARG, "batch is null");
-  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (auto& ln : b->lane) HIP_TRY(hipStreamSynchronize(ln.stream));
   return RSG_OK;
 }
 

@@ -149,16 +149,17 @@ hipError_t launch_velocity_f64(const VelocityArgs<double>& a, int block, hipStre
 hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStream_t s);
 hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t s);
 hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t s);
-// Batched launch: grid workgroups walk work = {n, flags, (instance, block)...}; lds
-// selects LDS staging (max_u: the largest staged set of any instance), cpt the entities
-// per thread (1, 2 or 4), km the coefficient form of every instance the table names (0
-// per-entity arrays, 1 class-table keys, 3 keys packed with the LDS links).  With graph != nullptr the launch is added to the graph as a
-// kernel node that depends on *node (when not null) and *node becomes the new node;
-// otherwise it is enqueued on s.
-hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* work, int k, int grid, int block,
+// Batched launch: grid workgroups walk table = n (instance, block) int32 pairs in launch
+// order, 8-byte aligned, (-1, 0) for an empty item; lds selects LDS staging (max_u: the
+// largest staged set of any instance), cpt the entities per thread (1, 2 or 4), km the
+// coefficient form of every instance the table names (0 per-entity arrays, 1 class-table
+// keys, 3 keys packed with the LDS links).  With graph != nullptr the launch is added to
+// the graph as a kernel node that depends on *node (when not null) and *node becomes the
+// new node; otherwise it is enqueued on s.
+hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* table, int n, int k, int grid, int block,
                             bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node);
-hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* work, int k, int grid, int block,
+hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* table, int n, int k, int grid, int block,
                             bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node);
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s);

@@ -74,21 +74,6 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
   return x * q + (x < r ? x : r) + j;
 }
 
-// Batch work-table position -> item.  flags bit 0 off: identity.  Otherwise, with
-// g = flags >> 8: g == 0 gives XCD x the x-th contiguous eighth of the table (as xcd_block);
-// g > 0 gives each XCD runs of g consecutive items in turn (run r of XCD x is items
-// [(8r + x) g, (8r + x + 1) g)), with the last n mod 8g items in place.
-__device__ __forceinline__ int batch_item(int p, int n, int flags) {
-  if (!(flags & 1)) return p;
-  const int g = flags >> 8;
-  if (g == 0) return xcd_block(p, n);
-  const int n8 = n - n % (8 * g);
-  if (p >= n8) return p;
-  const int x = p & 7, j = p >> 3;
-  const int run = j / g;
-  return (run * 8 + x) * g + (j - run * g);
-}
-
 __device__ __forceinline__ bool is_boxed(double v) { return __builtin_isnan(v); }
 __device__ __forceinline__ bool is_boxed(float v) { return __builtin_isnan(v); }
 __device__ __forceinline__ int unbox(double v) {
@@ -594,50 +579,63 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(S
   stress_body<T, LDS, U, CPT, KM>(a, a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x, smem);
 }
 
-// Batched launches: a persistent grid walks the work table (work[0] items of (instance,
-// block) after a two-int header whose second int holds flags) and reads each instance's
-// argument block from desc.  With flag bit 0 the item positions are remapped like the
-// single-instance block ids, so each XCD takes a contiguous run of the table (neighbouring
-// blocks of one instance share halo entities in that XCD's L2).
+// Batched launches walk a table of n (instance, block) items.  The host stores the items
+// in launch order (any XCD-aware reordering already applied) and pads the table with
+// (-1, 0) items that do nothing, so a workgroup reads its item and then the instance's
+// argument block from desc, with no table header in the chain of dependent loads.  The
+// default grid has one workgroup per item and no loop (a loop around the body costs the
+// FP64 velocity kernel a wave of occupancy); PERSIST grids stride over the table.
 // Step k of the current K-step block runs only for instances with k < kmax, so instances
 // of different lengths stop exactly at their own step count.  The table and the
-// descriptors live in device memory, so a captured graph of these launches stays valid
-// while instances join and leave.
+// descriptors live in device memory, so a graph of these launches stays valid while
+// instances join and leave.
 template <typename T, bool LDS, int U, int CPT, int KM>
-__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_batch_kernel(
-    const BatchDesc<T>* __restrict__ desc, const int32_t* __restrict__ work, int k) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int n_work = work[0];
-  const int flags = work[1];
-  const int2* items = reinterpret_cast<const int2*>(work + 2);
-  for (int p = blockIdx.x; p < n_work; p += gridDim.x) {
-    const int2 it = items[batch_item(p, n_work, flags)];
-    const BatchDesc<T>* d = desc + it.x;
-    if (k < d->kmax) {
-      VelocityArgs<T> a = d->v;
-      a.amp = d->amp_base + (size_t)k * d->nch;
-      a.trace = d->trace_base ? d->trace_base + (size_t)(d->trace_row0 + k) * d->row : nullptr;
-      velocity_body<T, LDS, U, CPT, KM>(a, it.y, smem);
-    }
-    __syncthreads();
-  }
+__device__ __forceinline__ void velocity_item(const BatchDesc<T>* __restrict__ desc, int2 it, int k,
+                                              unsigned char* smem) {
+  if (it.x < 0) return;
+  const BatchDesc<T>* d = desc + it.x;
+  if (k >= d->kmax) return;
+  VelocityArgs<T> a = d->v;
+  a.amp = d->amp_base + (size_t)k * d->nch;
+  a.trace = d->trace_base ? d->trace_base + (size_t)(d->trace_row0 + k) * d->row : nullptr;
+  velocity_body<T, LDS, U, CPT, KM>(a, it.y, smem);
 }
 
 template <typename T, bool LDS, int U, int CPT, int KM>
-__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_batch_kernel(
-    const BatchDesc<T>* __restrict__ desc, const int32_t* __restrict__ work, int k) {
+__device__ __forceinline__ void stress_item(const BatchDesc<T>* __restrict__ desc, int2 it, int k,
+                                            unsigned char* smem) {
+  if (it.x < 0) return;
+  const BatchDesc<T>* d = desc + it.x;
+  if (k >= d->kmax) return;
+  StressArgs<T> a = d->s;
+  a.amp = d->amp_base + (size_t)k * d->nch;
+  stress_body<T, LDS, U, CPT, KM>(a, it.y, smem);
+}
+
+template <typename T, bool LDS, int U, int CPT, int KM, bool PERSIST>
+__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_VELOCITY_ATTR void velocity_batch_kernel(
+    const BatchDesc<T>* __restrict__ desc, const int2* __restrict__ items, int n, int k) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int n_work = work[0];
-  const int flags = work[1];
-  const int2* items = reinterpret_cast<const int2*>(work + 2);
-  for (int p = blockIdx.x; p < n_work; p += gridDim.x) {
-    const int2 it = items[batch_item(p, n_work, flags)];
-    const BatchDesc<T>* d = desc + it.x;
-    if (k < d->kmax) {
-      StressArgs<T> a = d->s;
-      a.amp = d->amp_base + (size_t)k * d->nch;
-      stress_body<T, LDS, U, CPT, KM>(a, it.y, smem);
-    }
+  if (!PERSIST) {
+    if ((int)blockIdx.x < n) velocity_item<T, LDS, U, CPT, KM>(desc, items[blockIdx.x], k, smem);
+    return;
+  }
+  for (int p = blockIdx.x; p < n; p += gridDim.x) {
+    velocity_item<T, LDS, U, CPT, KM>(desc, items[p], k, smem);
+    __syncthreads();  // the next item reuses the LDS
+  }
+}
+
+template <typename T, bool LDS, int U, int CPT, int KM, bool PERSIST>
+__global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_batch_kernel(
+    const BatchDesc<T>* __restrict__ desc, const int2* __restrict__ items, int n, int k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (!PERSIST) {
+    if ((int)blockIdx.x < n) stress_item<T, LDS, U, CPT, KM>(desc, items[blockIdx.x], k, smem);
+    return;
+  }
+  for (int p = blockIdx.x; p < n; p += gridDim.x) {
+    stress_item<T, LDS, U, CPT, KM>(desc, items[p], k, smem);
     __syncthreads();
   }
 }
@@ -755,19 +753,24 @@ hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t 
   return launch_stress<float>(a, block, s);
 }
 template <typename T>
-hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* work, int k, int grid, int block, bool lds,
-                        int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
+hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* table, int n, int k, int grid, int block,
+                        bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                         hipGraphNode_t* node) {
-  if (grid <= 0) return hipSuccess;
+  if (grid <= 0 || n <= 0) return hipSuccess;
+  if (reinterpret_cast<uintptr_t>(table) % alignof(int2)) return hipErrorInvalidValue;
+  const int2* items = reinterpret_cast<const int2*>(table);
   if (block < 64 || block > RSG_MAX_BLOCK || (block % 64) != 0) return hipErrorInvalidValue;
   const size_t bytes = lds ? (size_t)(velocity ? 6 : 3) * max_u * sizeof(T) : 0;
+  const bool persist = grid != n;
   return with_variant(lds, max_u, block, cpt, km, [&](auto L, auto U, auto C, auto K) {
     constexpr bool l = decltype(L)::value;
     constexpr int u = decltype(U)::value, c = decltype(C)::value, m = decltype(K)::value;
+    const void* fn = velocity ? (persist ? reinterpret_cast<const void*>(&velocity_batch_kernel<T, l, u, c, m, true>)
+                                         : reinterpret_cast<const void*>(&velocity_batch_kernel<T, l, u, c, m, false>))
+                              : (persist ? reinterpret_cast<const void*>(&stress_batch_kernel<T, l, u, c, m, true>)
+                                         : reinterpret_cast<const void*>(&stress_batch_kernel<T, l, u, c, m, false>));
     if (graph) {
-      const void* fn = velocity ? reinterpret_cast<const void*>(&velocity_batch_kernel<T, l, u, c, m>)
-                                : reinterpret_cast<const void*>(&stress_batch_kernel<T, l, u, c, m>);
-      void* args[3] = {(void*)&desc, (void*)&work, (void*)&k};
+      void* args[4] = {(void*)&desc, (void*)&items, (void*)&n, (void*)&k};
       hipKernelNodeParams p{};
       p.func = const_cast<void*>(fn);
       p.gridDim = dim3(grid);
@@ -778,23 +781,20 @@ hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* work, int k, in
       const hipGraphNode_t prev = *node;
       return hipGraphAddKernelNode(node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &p);
     }
-    if (velocity)
-      hipLaunchKernelGGL((velocity_batch_kernel<T, l, u, c, m>), dim3(grid), dim3(block), bytes, s, desc, work, k);
-    else
-      hipLaunchKernelGGL((stress_batch_kernel<T, l, u, c, m>), dim3(grid), dim3(block), bytes, s, desc, work, k);
-    return hipGetLastError();
+    void* args[4] = {(void*)&desc, (void*)&items, (void*)&n, (void*)&k};
+    return hipLaunchKernel(fn, dim3(grid), dim3(block), args, bytes, s);
   });
 }
 
-hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* work, int k, int grid, int block,
+hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* table, int n, int k, int grid, int block,
                             bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node) {
-  return launch_batch<double>(d, work, k, grid, block, lds, max_u, cpt, km, velocity, s, graph, node);
+  return launch_batch<double>(d, table, n, k, grid, block, lds, max_u, cpt, km, velocity, s, graph, node);
 }
-hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* work, int k, int grid, int block,
+hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* table, int n, int k, int grid, int block,
                             bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node) {
-  return launch_batch<float>(d, work, k, grid, block, lds, max_u, cpt, km, velocity, s, graph, node);
+  return launch_batch<float>(d, table, n, k, grid, block, lds, max_u, cpt, km, velocity, s, graph, node);
 }
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(energy_kernel<double>, dim3(nblocks), dim3(256), 0, s, a);

@@ -25,7 +25,7 @@ LIB_PATH = os.environ.get("PANNDE_RSG_LIB") or os.path.join(HERE, "libpannde_rsg
 RSG_OK = 0
 RSG_FP64, RSG_FP32 = 0, 1
 RSG_FORCE, RSG_MOMENT = 0, 1
-RSG_BATCH_GROUP_BYTES, RSG_BATCH_XCD_RUN = 1, 2
+RSG_BATCH_GROUP_BYTES, RSG_BATCH_XCD_RUN, RSG_BATCH_LANES = 1, 2, 3
 STATE_V = ("Vx", "Vy", "Vz")
 STATE_S = ("Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy")
 
@@ -358,10 +358,10 @@ class DeviceSolver:
 class Batch:
     """Many DeviceSolver instances advanced together (rsg_batch_*): one launch per kernel
     per step over each group of instances, K-step blocks replayed from a HIP graph.
-    group_bytes / xcd_run set the scheduling options (None keeps the library default)."""
+    group_bytes / xcd_run / lanes set the scheduling options (None keeps the library default)."""
 
     def __init__(self, precision=RSG_FP64, steps_per_block=50, grid=0, use_graph=True, group_bytes=None,
-                 xcd_run=None):
+                 xcd_run=None, lanes=None):
         h = ctypes.c_void_p()
         _check(lib().rsg_batch_create(int(precision), int(steps_per_block), int(grid), 1 if use_graph else 0,
                                       ctypes.byref(h)))
@@ -371,6 +371,8 @@ class Batch:
             self.set_option(RSG_BATCH_GROUP_BYTES, group_bytes)
         if xcd_run is not None:
             self.set_option(RSG_BATCH_XCD_RUN, xcd_run)
+        if lanes is not None:
+            self.set_option(RSG_BATCH_LANES, lanes)
 
     def set_option(self, option: int, value: int):
         _check(lib().rsg_batch_set_option(self._h, int(option), int(value)))

@@ -60,25 +60,28 @@ def test_batch_equals_alone(precision, use_graph):
     b.close()
 
 
-# (grid, group_bytes, xcd_run): one group / one instance per group / a budget that splits
-# the mix unevenly; table order, one eighth per XCD, runs of 3 and 64 items; persistent grids
-SCHEDULES = [(0, 0, -1), (0, 1, 0), (0, 200_000, 3), (-1, 1 << 27, 64), (64, 0, 64), (37, 1, 0)]
+# (grid, group_bytes, xcd_run, lanes): one group / one instance per group / a budget that
+# splits the mix unevenly; table order, one eighth per XCD, runs of 3 and 64 items;
+# persistent grids; one to four lanes
+SCHEDULES = [(0, 0, -1, 1), (0, 1, 0, 4), (0, 200_000, 3, 2), (-1, 1 << 27, 64, 3), (64, 0, 64, 2),
+             (37, 1, 0, 3), (0, 1, 64, 1)]
 
 
 @pytest.mark.parametrize("precision", [rsg.RSG_FP64, rsg.RSG_FP32])
 def test_schedules_agree(precision):
-    """Group budget, XCD order and grid form only reorder independent work."""
+    """Group budget, XCD order, grid form and lanes only reorder independent work."""
     jobs = mix()
     solo = [alone(c, n, precision) for c, n in jobs]
-    for grid, group_bytes, xcd_run in SCHEDULES:
-        b = rsg.Batch(precision, steps_per_block=9, grid=grid, group_bytes=group_bytes, xcd_run=xcd_run)
+    for grid, group_bytes, xcd_run, lanes in SCHEDULES:
+        b = rsg.Batch(precision, steps_per_block=9, grid=grid, group_bytes=group_bytes, xcd_run=xcd_run,
+                      lanes=lanes)
         solvers = [rsg.DeviceSolver.from_case(c, precision) for c, _ in jobs]
         slots = [b.add(s, n) for s, (_, n) in zip(solvers, jobs)]
         assert b.run(1000) == 0
         b.synchronize()
         for s, (c, n), ref in zip(solvers, jobs, solo):
             V, S = s.get_state()
-            tag = "%s grid=%d group=%d xcd=%d" % (c.name, grid, group_bytes, xcd_run)
+            tag = "%s grid=%d group=%d xcd=%d lanes=%d" % (c.name, grid, group_bytes, xcd_run, lanes)
             assert np.array_equal(V, ref[0]) and np.array_equal(S, ref[1]), tag
             assert np.array_equal(s.traces(), ref[3]), tag
         for slot in slots:
@@ -94,6 +97,9 @@ def test_batch_option_validation():
         b.set_option(rsg.RSG_BATCH_GROUP_BYTES, -1)
     with pytest.raises(rsg.RSGError, match="xcd run"):
         b.set_option(rsg.RSG_BATCH_XCD_RUN, -2)
+    for bad in (0, 5):
+        with pytest.raises(rsg.RSGError, match="lanes"):
+            b.set_option(rsg.RSG_BATCH_LANES, bad)
     with pytest.raises(rsg.RSGError, match="unknown batch option"):
         b.set_option(99, 0)
     b.close()
