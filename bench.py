@@ -15,12 +15,14 @@ the reference solver itself (oracle/_ref/ref_rsg, built from the reference heade
 mpirun on this host's cores, on a bounded sample of the same job; fp32 = the mixed-
 precision form on the same job with its sensor-trace error against FP64; bracket = the
 cracked Ti-6Al-4V L-bracket (9.4 M cells, 4000 steps) in both forms with the FP32 trace error;
-batched = 64
-independent instances (32^3..128^3 cells, 1500 steps, 50-step graph blocks) advanced
-together in FP32, every instance's traces checked against the same batch run in FP64;
-served = a bounded sample (the first 120 requests) of the seeded inspection stream (cracked brackets and plates,
-0.5M-8M cells, 1000-4000 steps, Poisson arrivals) through the continuous-batching
-server, with solves/hr, p50/p99 latency and the FP64 spot checks.
+batched = 64 independent instances (32^3..128^3 cells, 1500 steps, 50-step graph blocks)
+advanced together in FP32, every instance's traces checked against the same batch run in
+FP64; served = a bounded sample (the first 120 requests) of the seeded inspection stream
+(cracked brackets and plates, 0.5M-8M cells, 1000-4000 steps, Poisson arrivals at 4 per
+second) through the continuous-batching server, with solves/hr, p50/p99 latency and the
+FP64 spot checks; served_saturated = the same requests arriving at 40 per second, faster
+than the server retires them, so solves/hr is its capacity (latency then includes the
+queue).
 """
 import argparse
 import json
@@ -196,6 +198,8 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=4)
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-saturated-rate", type=float, default=40.0,
+                    help="arrival rate of the capacity leg (0: skip it)")
     ap.add_argument("--serve-seed", type=int, default=9)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -293,6 +297,10 @@ def main():
         log("bench: served stream, %d requests at %.2f per second" % (args.serve_requests, args.serve_rate))
         out["served"] = served(rsg, args.serve_requests, args.serve_rate, args.serve_seed)
         out["served"]["ratio_served_vs_fp64_plain"] = round(out["served"]["gcell_steps_per_s"] / value, 3)
+        if args.serve_saturated_rate > 0:
+            log("bench: served capacity, %d requests at %.2f per second (arrivals outpace the server)"
+                % (args.serve_requests, args.serve_saturated_rate))
+            out["served_saturated"] = served(rsg, args.serve_requests, args.serve_saturated_rate, args.serve_seed)
 
     if not args.no_cpu_baseline:
         ranks = args.cpu_ranks or max(1, min(16, os.cpu_count() or 1))

@@ -97,17 +97,24 @@ def structured_mesh(nx, ny, nz, dx, dy, dz, keep=None, origin=(0.0, 0.0, 0.0)):
         for dj in (0, 1):
             for dk in (0, 1):
                 used[di:di + nx, dj:dj + ny, dk:dk + nz] |= keep
-    nid = np.full(used.shape, -1, dtype=np.int64)
-    nid[used] = np.arange(int(used.sum()))  # C order: x outer, z inner
-    ii, jj, kk = np.nonzero(used)
+    # flat indexing throughout (C order: x outer, z inner)
+    sy, sx = nz + 1, (ny + 1) * (nz + 1)
+    flat = np.flatnonzero(used)
+    nid = np.full(used.size, -1, dtype=np.int32)
+    nid[flat] = np.arange(flat.shape[0], dtype=np.int32)
+    ii, rem = np.divmod(flat, sx)
+    jj, kk = np.divmod(rem, sy)
     x = origin[0] + ii * dx
     y = origin[1] + jj * dy
     z = origin[2] + kk * dz
-    ci, cj, ck = np.nonzero(keep)
+    cflat = np.flatnonzero(keep)
+    ci, rem = np.divmod(cflat, ny * nz)
+    cj, ck = np.divmod(rem, nz)
+    base = ci * sx + cj * sy + ck
     offs = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
     cells = np.empty((ci.shape[0], 8), dtype=np.int32)
     for v, (a, b, c) in enumerate(offs):
-        cells[:, v] = nid[ci + a, cj + b, ck + c]
+        cells[:, v] = nid[base + (a * sx + b * sy + c)]
     return (x.astype(np.float64), y.astype(np.float64), z.astype(np.float64), cells,
             np.stack([ci, cj, ck], axis=1))
 
@@ -168,6 +175,12 @@ def rotate_c_about_z(c6, degrees):
 def cij_cell_fields(c6_per_cell):
     """21 CELL fields from an [nc, 6, 6] stiffness array."""
     return [Field(FIELD_CELL, np.ascontiguousarray(c6_per_cell[:, i, j], dtype=np.float64)) for i, j in CIJ_PAIRS]
+
+
+def cij_material_fields(c6_per_material, material_of_cell):
+    """21 CELL fields from [nmat, 6, 6] stiffnesses and each cell's material index."""
+    return [Field(FIELD_CELL, np.ascontiguousarray(c6_per_material[:, i, j], dtype=np.float64)[material_of_cell])
+            for i, j in CIJ_PAIRS]
 
 
 def max_wave_speed(c6, rho):
@@ -281,7 +294,7 @@ def cfrp_plate(seed=2, nx=256, ny=256, cells_per_ply=4, dx=5.0e-5, nsteps=2000, 
     if crack is not None:
         x, y, z, cells, n_dup = surface_crack(x, y, z, cells, ijk, (dx, dx, dx), crack)
     ply_c = np.stack([rotate_c_about_z(cfrp_ply_c(), a) for a in CFRP_LAYUP])
-    c_cells = ply_c[ijk[:, 2] // cells_per_ply]
+    ply_of_cell = ijk[:, 2] // cells_per_ply
     rho = 1580.0
     if renumber:
         perm = rng.permutation(x.shape[0])
@@ -291,7 +304,7 @@ def cfrp_plate(seed=2, nx=256, ny=256, cells_per_ply=4, dx=5.0e-5, nsteps=2000, 
     t, v = tone_burst(2.0e6, 5, dt)
     disc = Disc(2, center, xd_radius * scale, dx, t, v)
     return Case(name="cfrp_plate_%dx%dx%d" % (nx, ny, nz), x=x, y=y, z=z, cells=cells,
-                rho=Field(FIELD_CELL, np.full(cells.shape[0], rho)), cij=cij_cell_fields(c_cells),
+                rho=Field(FIELD_CELL, np.full(cells.shape[0], rho)), cij=cij_material_fields(ply_c, ply_of_cell),
                 dt=dt, nsteps=nsteps, force=[disc],
                 sensors=seeded_sensors(rng, top_face_nodes(z), n_sensors), seed=seed,
                 meta=dict(crack=crack, n_dup=n_dup, holes=holes))
