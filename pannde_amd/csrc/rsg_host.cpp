@@ -313,6 +313,31 @@ double field_at_cell(const rsg_field_desc& f, const int32_t* cells, int32_t c) {
   return f.type == RSG_FIELD_CONSTANT ? f.data[0] : f.data[c];
 }
 
+// Set-up loops over n entities run in up to 8 contiguous chunks of at least 64 Ki entities.
+int chunk_count(int64_t n) {
+  constexpr int64_t kMinChunk = 1 << 16;
+  const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+  return (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(8, hw), (n + kMinChunk - 1) / kMinChunk));
+}
+
+// Runs fn(chunk, begin, end) for chunk_count(n) contiguous chunks of [0, n), one thread each
+// (the caller takes chunk 0).  fn must not throw; the chunks write disjoint data.
+template <typename F>
+void parallel_chunks(int64_t n, F&& fn) {
+  const int t = chunk_count(n);
+  const int64_t step = (n + t - 1) / t;
+  std::vector<std::thread> workers;
+  for (int w = 1; w < t; w++)
+    workers.emplace_back([&fn, n, step, w]() { fn(w, std::min(n, w * step), std::min(n, (w + 1) * step)); });
+  fn(0, (int64_t)0, std::min(n, step));
+  for (auto& w : workers) w.join();
+}
+
+template <typename F>
+void parallel_ranges(int64_t n, F&& fn) {
+  parallel_chunks(n, [&fn](int, int64_t b, int64_t e) { fn(b, e); });
+}
+
 // Locality order: the point quantised to the finest spacing, (qx, qy, qz), sorted by
 // bricks of tile[0] x tile[1] x tile[2] (brick z, brick y, brick x, then z, y, x inside
 // the brick).  tile = {huge, 1, 1} is plain (z, y, x) lexicographic order: x-runs are
@@ -475,39 +500,105 @@ struct Classifier {
 // of first appearance.  On success uploads the material table (rows of ROW values) and
 // the [3][kAxisClasses] spacing table and sets *materials; leaves it at -1 when the mesh
 // has more than kMaxMaterials materials or kAxisClasses spacings along an axis.
+// Contiguous chunks are classified on their own threads, then their classes are merged in
+// chunk order, which numbers every class by its first appearance exactly as one pass
+// would; entity() must be safe to call from several threads.
 template <typename T, int W, int ROW, typename F>
 int make_keys(rsg_instance* h, int32_t n, F&& entity, std::vector<uint32_t>& keys, void** mat_table,
               void** axis_table, int32_t* materials) {
   *materials = -1;
+  keys.resize(n);
+  struct Chunk {
+    Classifier<T, W> mats{(size_t)rsg::kMaxMaterials};
+    std::vector<T> axv[3];  // spacing classes in order of first appearance in the chunk
+    bool overflow = false;
+    std::vector<int32_t> mat_map, ax_map[3];  // chunk class -> merged class
+  };
+  std::vector<Chunk> chunks(chunk_count(n));
+  parallel_chunks(n, [&](int w, int64_t b, int64_t e) {
+    Chunk& ch = chunks[w];
+    std::unordered_map<uint64_t, int> ax[3];
+    uint64_t last_bits[3] = {0, 0, 0};
+    int last_class[3] = {-1, -1, -1};  // the previous entity's spacing classes
+    T m[W], sp[3];
+    for (int32_t i = (int32_t)b; i < (int32_t)e; i++) {
+      entity(i, m, sp);
+      const int32_t mc = ch.mats.add(m);
+      if (mc < 0) {
+        ch.overflow = true;
+        return;
+      }
+      int ic[3];
+      for (int d = 0; d < 3; d++) {
+        uint64_t bits = 0;
+        memcpy(&bits, &sp[d], sizeof(T));
+        if (last_class[d] >= 0 && bits == last_bits[d]) {
+          ic[d] = last_class[d];
+          continue;
+        }
+        auto it = ax[d].find(bits);
+        if (it == ax[d].end()) {
+          if ((int)ch.axv[d].size() >= rsg::kAxisClasses) {
+            ch.overflow = true;
+            return;
+          }
+          it = ax[d].emplace(bits, (int)ch.axv[d].size()).first;
+          ch.axv[d].push_back(sp[d]);
+        }
+        ic[d] = it->second;
+        last_bits[d] = bits;
+        last_class[d] = ic[d];
+      }
+      keys[i] = rsg::make_key(mc, ic[0], ic[1], ic[2]);
+    }
+  });
+  // merge in chunk order
   Classifier<T, W> mats(rsg::kMaxMaterials);
   std::unordered_map<uint64_t, int> ax[3];
   std::vector<T> axv[3];
-  keys.resize(n);
-  T m[W], sp[3];
-  for (int32_t i = 0; i < n; i++) {
-    entity(i, m, sp);
-    const int32_t mc = mats.add(m);
-    if (mc < 0) {
+  for (auto& ch : chunks) {
+    if (ch.overflow) {
       keys.clear();
       return RSG_OK;
     }
-    int ic[3];
-    for (int d = 0; d < 3; d++) {
-      uint64_t bits = 0;
-      memcpy(&bits, &sp[d], sizeof(T));
-      auto it = ax[d].find(bits);
-      if (it == ax[d].end()) {
-        if ((int)axv[d].size() >= rsg::kAxisClasses) {
-          keys.clear();
-          return RSG_OK;
-        }
-        it = ax[d].emplace(bits, (int)axv[d].size()).first;
-        axv[d].push_back(sp[d]);
+    const size_t nm = ch.mats.table.size() / W;
+    ch.mat_map.resize(nm);
+    for (size_t c = 0; c < nm; c++) {
+      T row[W];
+      memcpy(row, &ch.mats.table[c * W], sizeof(row));
+      const int32_t g = mats.add(row);
+      if (g < 0) {
+        keys.clear();
+        return RSG_OK;
       }
-      ic[d] = it->second;
+      ch.mat_map[c] = g;
     }
-    keys[i] = rsg::make_key(mc, ic[0], ic[1], ic[2]);
+    for (int d = 0; d < 3; d++) {
+      ch.ax_map[d].resize(ch.axv[d].size());
+      for (size_t k = 0; k < ch.axv[d].size(); k++) {
+        uint64_t bits = 0;
+        memcpy(&bits, &ch.axv[d][k], sizeof(T));
+        auto it = ax[d].find(bits);
+        if (it == ax[d].end()) {
+          if ((int)axv[d].size() >= rsg::kAxisClasses) {
+            keys.clear();
+            return RSG_OK;
+          }
+          it = ax[d].emplace(bits, (int)axv[d].size()).first;
+          axv[d].push_back(ch.axv[d][k]);
+        }
+        ch.ax_map[d][k] = it->second;
+      }
+    }
   }
+  parallel_chunks(n, [&](int w, int64_t b, int64_t e) {
+    const Chunk& ch = chunks[w];
+    for (int64_t i = b; i < e; i++) {
+      const uint32_t k = keys[i];
+      keys[i] = rsg::make_key(ch.mat_map[k >> rsg::kMatShift], ch.ax_map[0][(k >> 14) & 0x7F],
+                              ch.ax_map[1][(k >> 7) & 0x7F], ch.ax_map[2][k & 0x7F]);
+    }
+  });
   const size_t nm = mats.table.size() / W;
   std::vector<T> mt(nm * ROW, T(0)), at((size_t)3 * rsg::kAxisClasses, T(0));
   for (size_t c = 0; c < nm; c++)
@@ -963,10 +1054,14 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     for (int d = 0; d < 3; d++) dcell[d][c] = X[d][b] - X[d][a];
   }
   CREATE_PHASE("cellprep");
-  // RightHex::nodePrep
+  // RightHex::nodePrep (in parallel; the first node whose spacing fails is reported)
   std::vector<double> dnode[3];
   for (int d = 0; d < 3; d++) dnode[d].resize(nn);
-  for (int32_t n = 0; n < nn; n++) {
+  std::mutex bad_mu;
+  int64_t bad_node = -1;
+  int bad_axis = 0;
+  parallel_ranges(nn, [&](int64_t n0, int64_t n1) {
+  for (int32_t n = (int32_t)n0; n < (int32_t)n1; n++) {
     double pn[3], lo[3], hi[3];
     for (int d = 0; d < 3; d++) pn[d] = lo[d] = hi[d] = X[d][n];
     for (int kb = 0; kb < 8; kb++) {
@@ -985,11 +1080,22 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
       if (lo[d] == pn[d]) lo[d] = 2. * pn[d] - hi[d];
       if (hi[d] == pn[d]) hi[d] = 2. * pn[d] - lo[d];
     }
-    static const char* kAxis[3] = {"DX", "DY", "DZ"};
     for (int d = 0; d < 3; d++) {
       dnode[d][n] = 0.5 * (hi[d] - lo[d]);
-      if (0 >= dnode[d][n]) return fail(RSG_ERR_MESH, "%s (node %d)", kAxis[d], n);
+      if (0 >= dnode[d][n]) {
+        std::lock_guard<std::mutex> lock(bad_mu);
+        if (bad_node < 0 || n < bad_node) {
+          bad_node = n;
+          bad_axis = d;
+        }
+        return;
+      }
     }
+  }
+  });
+  if (bad_node >= 0) {
+    static const char* kAxis[3] = {"DX", "DY", "DZ"};
+    return fail(RSG_ERR_MESH, "%s (node %d)", kAxis[bad_axis], (int)bad_node);
   }
 
   CREATE_PHASE("nodeprep");
@@ -1006,18 +1112,33 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   }
   {
     std::vector<double> cx(nc), cy(nc), cz(nc);
-    for (int32_t c = 0; c < nc; c++) {
-      double s[3] = {0, 0, 0};
-      for (int v = 0; v < 8; v++)
-        for (int d = 0; d < 3; d++) s[d] += X[d][h->cells[(size_t)c * 8 + v]];
-      cx[c] = s[0] / 8;
-      cy[c] = s[1] / 8;
-      cz[c] = s[2] / 8;
-    }
+    parallel_ranges(nc, [&](int64_t c0, int64_t c1) {
+      for (int32_t c = (int32_t)c0; c < (int32_t)c1; c++) {
+        double s[3] = {0, 0, 0};
+        for (int v = 0; v < 8; v++)
+          for (int d = 0; d < 3; d++) s[d] += X[d][h->cells[(size_t)c * 8 + v]];
+        cx[c] = s[0] / 8;
+        cy[c] = s[1] / 8;
+        cz[c] = s[2] / 8;
+      }
+    });
     double clo[3] = {lo[0] + 0.5 * hmin[0], lo[1] + 0.5 * hmin[1], lo[2] + 0.5 * hmin[2]};
-    h->cell_of = locality_order(nc, cx, cy, cz, hmin, clo, h->tile);
     std::vector<double> nx(X[0], X[0] + nn), ny(X[1], X[1] + nn), nz(X[2], X[2] + nn);
-    h->node_of = locality_order(nn, nx, ny, nz, hmin, lo, h->tile);
+    bool cells_ok = true, nodes_ok = true;
+    std::thread cell_sort([&]() {  // the two orders are independent
+      try {
+        h->cell_of = locality_order(nc, cx, cy, cz, hmin, clo, h->tile);
+      } catch (const std::bad_alloc&) {
+        cells_ok = false;
+      }
+    });
+    try {
+      h->node_of = locality_order(nn, nx, ny, nz, hmin, lo, h->tile);
+    } catch (const std::bad_alloc&) {
+      nodes_ok = false;
+    }
+    cell_sort.join();
+    if (!cells_ok || !nodes_ok) return fail(RSG_ERR_NOMEM, "host allocation failed");
   }
   h->cell_at.assign(nc, -1);
   h->node_at.assign(nn, -1);
@@ -1041,19 +1162,31 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     std::vector<double> idt(nn), w(nn), dn[3];
     for (int d = 0; d < 3; d++) dn[d].resize(nn);
     std::vector<int32_t> conn((size_t)nn * 8);
-    for (int32_t i = 0; i < nn; i++) {
-      const int32_t n = h->node_of[i];
-      const double rho = field_at_node(model->density, h->node_cells.data(), n);
-      idt[i] = h->dt / (rho);
-      if (!(idt[i] == idt[i]) || std::isinf(idt[i]))
-        return fail(RSG_ERR_ARG, "dt/density is not finite at node %d (density %g)", n, rho);
-      w[i] = rho * 8.0 * dnode[0][n] * dnode[1][n] * dnode[2][n];
-      for (int d = 0; d < 3; d++)
-        dn[d][i] = precision == RSG_FP64 ? dnode[d][n] : 0.25 / dnode[d][n];
-      for (int k = 0; k < 8; k++) {
-        const int32_t c = h->node_cells[(size_t)n * 8 + k];
-        conn[(size_t)i * 8 + k] = c < 0 ? -1 : h->cell_at[c];
+    std::mutex bad_mu;
+    int64_t bad = -1;  // first internal node whose dt/density is not finite
+    parallel_ranges(nn, [&](int64_t i0, int64_t i1) {
+      for (int32_t i = (int32_t)i0; i < (int32_t)i1; i++) {
+        const int32_t n = h->node_of[i];
+        const double rho = field_at_node(model->density, h->node_cells.data(), n);
+        idt[i] = h->dt / (rho);
+        if (!(idt[i] == idt[i]) || std::isinf(idt[i])) {
+          std::lock_guard<std::mutex> lock(bad_mu);
+          if (bad < 0 || i < bad) bad = i;
+          return;
+        }
+        w[i] = rho * 8.0 * dnode[0][n] * dnode[1][n] * dnode[2][n];
+        for (int d = 0; d < 3; d++)
+          dn[d][i] = precision == RSG_FP64 ? dnode[d][n] : 0.25 / dnode[d][n];
+        for (int k = 0; k < 8; k++) {
+          const int32_t c = h->node_cells[(size_t)n * 8 + k];
+          conn[(size_t)i * 8 + k] = c < 0 ? -1 : h->cell_at[c];
+        }
       }
+    });
+    if (bad >= 0) {
+      const int32_t n = h->node_of[bad];
+      return fail(RSG_ERR_ARG, "dt/density is not finite at node %d (density %g)", n,
+                  field_at_node(model->density, h->node_cells.data(), n));
     }
     h->idt_h = idt;
     if (h->classes) {  // material: dt/rho; spacings as the per-node arrays hold them
@@ -1107,13 +1240,15 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     std::vector<int32_t> conn((size_t)nc * 8);
     const int64_t ncp = ((int64_t)nc + 63) / 64 * 64;
     h->ldc = h->c_blocked ? 0 : ncp;
-    for (int32_t i = 0; i < nc; i++) {
-      const int32_t c = h->cell_of[i];
-      for (int d = 0; d < 3; d++)
-        dc[d][i] = precision == RSG_FP64 ? dcell[d][c] : 0.25 / dcell[d][c];
-      w[i] = dcell[0][c] * dcell[1][c] * dcell[2][c];
-      for (int k = 0; k < 8; k++) conn[(size_t)i * 8 + k] = h->node_at[h->cells[(size_t)c * 8 + k]];
-    }
+    parallel_ranges(nc, [&](int64_t i0, int64_t i1) {
+      for (int32_t i = (int32_t)i0; i < (int32_t)i1; i++) {
+        const int32_t c = h->cell_of[i];
+        for (int d = 0; d < 3; d++)
+          dc[d][i] = precision == RSG_FP64 ? dcell[d][c] : 0.25 / dcell[d][c];
+        w[i] = dcell[0][c] * dcell[1][c] * dcell[2][c];
+        for (int k = 0; k < 8; k++) conn[(size_t)i * 8 + k] = h->node_at[h->cells[(size_t)c * 8 + k]];
+      }
+    });
     h->dcx_h = dc[0];
     // C entry j of internal cell i as the kernels read it: FP64 C_IJ, FP32 dt*C_IJ
     auto c_entry = [&](int32_t i, int j) {

@@ -85,3 +85,19 @@ def test_create_reports_reference_dx_error():
         rsg.DeviceSolver(np.zeros_like(x), y, z, cells, cases.Field(cases.FIELD_CONSTANT, np.ones(1)),
                          cases.cij_cell_fields(np.broadcast_to(c6, (1, 6, 6))), 1e-9)
     assert e.value.code == -2 and "DX" in str(e.value)
+
+
+@pytest.mark.parametrize("axis", ["DX", "DY", "DZ"])
+def test_create_reports_the_first_bad_node_on_a_large_mesh(axis):
+    """Node spacings are computed in parallel chunks; the error still names the first node."""
+    x, y, z, cells, _ = cases.structured_mesh(60, 60, 24, 1.0, 1.0, 1.0)  # 90,585 nodes
+    assert x.shape[0] > 1 << 16
+    xyz = [x, y, z]
+    k = ["DX", "DY", "DZ"].index(axis)
+    xyz[k] = np.zeros_like(xyz[k])
+    c6 = cases.isotropic_c(1e9, 1e9)
+    with pytest.raises(rsg.RSGError) as e:
+        rsg.DeviceSolver(*xyz, cells, cases.Field(cases.FIELD_CONSTANT, np.ones(1)),
+                         [cases.Field(cases.FIELD_CONSTANT, np.array([c6[i, j]])) for i, j in cases.CIJ_PAIRS],
+                         1e-9)
+    assert e.value.code == -2 and "%s (node 0)" % axis in str(e.value)
