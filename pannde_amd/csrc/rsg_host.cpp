@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -320,17 +321,37 @@ int chunk_count(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(8, hw), (n + kMinChunk - 1) / kMinChunk));
 }
 
-// Runs fn(chunk, begin, end) for chunk_count(n) contiguous chunks of [0, n), one thread each
-// (the caller takes chunk 0).  fn must not throw; the chunks write disjoint data.
+// Runs fn(w) for w in [0, t), one thread each (the caller takes w = 0); an exception of any
+// task (std::bad_alloc) is rethrown in the caller once all have ended.
+template <typename F>
+void parallel_tasks(int t, F&& fn) {
+  std::vector<std::exception_ptr> errs((size_t)t);
+  std::vector<std::thread> workers;
+  for (int w = 1; w < t; w++)
+    workers.emplace_back([&fn, &errs, w]() {
+      try {
+        fn(w);
+      } catch (...) {
+        errs[w] = std::current_exception();
+      }
+    });
+  try {
+    fn(0);
+  } catch (...) {
+    errs[0] = std::current_exception();
+  }
+  for (auto& w : workers) w.join();
+  for (auto& e : errs)
+    if (e) std::rethrow_exception(e);
+}
+
+// Runs fn(chunk, begin, end) for chunk_count(n) contiguous chunks of [0, n), one thread each.
+// The chunks must write disjoint data.
 template <typename F>
 void parallel_chunks(int64_t n, F&& fn) {
   const int t = chunk_count(n);
   const int64_t step = (n + t - 1) / t;
-  std::vector<std::thread> workers;
-  for (int w = 1; w < t; w++)
-    workers.emplace_back([&fn, n, step, w]() { fn(w, std::min(n, w * step), std::min(n, (w + 1) * step)); });
-  fn(0, (int64_t)0, std::min(n, step));
-  for (auto& w : workers) w.join();
+  parallel_tasks(t, [&fn, n, step](int w) { fn(w, std::min(n, w * step), std::min(n, (w + 1) * step)); });
 }
 
 template <typename F>
@@ -426,31 +447,56 @@ bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other
   const int32_t nb = (n + block - 1) / block;
   p.u_off.assign((size_t)nb + 1, 0);
   p.ulist.clear();
-  p.ulist.reserve((size_t)n * 2);
   p.lconn.assign((size_t)n * 8, 0xFFFF);
   p.max_u = 0;
-  std::vector<int32_t> seen((size_t)n_other, -1), local((size_t)n_other);
-  std::vector<int32_t> ids;
-  ids.reserve((size_t)block * 8);
-  for (int32_t b = 0; b < nb; b++) {
-    const int32_t lo = b * block, hi = std::min(n, lo + block);
-    ids.clear();
-    for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++) {
-      const int32_t g = conn[i];
-      if (g >= 0 && seen[g] != b) {
-        seen[g] = b;
-        ids.push_back(g);
+  // contiguous ranges of blocks on their own threads, each with its own generation-stamp set;
+  // the per-range lists are joined in block order
+  struct Range {
+    std::vector<int32_t> ulist;
+    int32_t max_u = 0;
+    bool ok = true;
+  };
+  std::vector<Range> ranges(chunk_count(n));
+  const int64_t nr = (int64_t)ranges.size(), step = (nb + nr - 1) / nr;
+  parallel_tasks((int)nr, [&](int w) {
+    Range& r = ranges[w];
+    const int32_t b0 = (int32_t)std::min<int64_t>(nb, w * step), b1 = (int32_t)std::min<int64_t>(nb, (w + 1) * step);
+    std::vector<int32_t> seen((size_t)n_other, -1), local((size_t)n_other);
+    std::vector<int32_t> ids;
+    ids.reserve((size_t)block * 8);
+    r.ulist.reserve((size_t)(b1 - b0) * block * 2);
+    for (int32_t b = b0; b < b1; b++) {
+      const int32_t lo = b * block, hi = std::min(n, lo + block);
+      ids.clear();
+      for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++) {
+        const int32_t g = conn[i];
+        if (g >= 0 && seen[g] != b) {
+          seen[g] = b;
+          ids.push_back(g);
+        }
       }
+      std::sort(ids.begin(), ids.end());
+      if (ids.size() >= 0xFFFF) {
+        r.ok = false;
+        return;
+      }
+      for (size_t k = 0; k < ids.size(); k++) local[ids[k]] = (int32_t)k;
+      for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++)
+        if (conn[i] >= 0) p.lconn[i] = (uint16_t)local[conn[i]];
+      r.ulist.insert(r.ulist.end(), ids.begin(), ids.end());
+      p.u_off[b + 1] = (int32_t)ids.size();  // sizes for now, offsets below
+      r.max_u = std::max(r.max_u, (int32_t)ids.size());
     }
-    std::sort(ids.begin(), ids.end());
-    if (ids.size() >= 0xFFFF) return false;
-    for (size_t k = 0; k < ids.size(); k++) local[ids[k]] = (int32_t)k;
-    for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++)
-      if (conn[i] >= 0) p.lconn[i] = (uint16_t)local[conn[i]];
-    p.ulist.insert(p.ulist.end(), ids.begin(), ids.end());
-    p.u_off[b + 1] = (int32_t)p.ulist.size();
-    p.max_u = std::max(p.max_u, (int32_t)ids.size());
+  });
+  size_t total = 0;
+  for (auto& r : ranges) {
+    if (!r.ok) return false;
+    total += r.ulist.size();
+    p.max_u = std::max(p.max_u, r.max_u);
   }
+  for (int32_t b = 0; b < nb; b++) p.u_off[b + 1] += p.u_off[b];
+  p.ulist.reserve(total);
+  for (auto& r : ranges) p.ulist.insert(p.ulist.end(), r.ulist.begin(), r.ulist.end());
   return true;
 }
 
