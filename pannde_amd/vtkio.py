@@ -3,7 +3,9 @@
 Reads the unstructured-grid job files the reference's drivers consume
 (modules/VTKIO/include/internal/VTKRead.hpp:115-297, through VTK's XML reader) and writes the
 solution and table files they produce (VTKWrite.hpp:200-232: ``<base>_<k>.vtu`` grids,
-``<name>.vtt`` / ``<name>_<k>.vtt`` tables).  Supported, as VTK's XML format defines them:
+``<name>.vtt`` / ``<name>_<k>.vtt`` tables; on several processes the partitioned ``.pvtu`` /
+``.pvtt`` summaries with ``<base>_<piece>`` pieces, VTKPWrite.hpp:100-137).  Supported, as VTK's
+XML format defines them:
 
 * ``ascii``, inline ``binary`` (base64) and ``appended`` data (raw or base64);
 * ``header_type`` UInt32 or UInt64, ``byte_order`` LittleEndian or BigEndian;
@@ -309,6 +311,94 @@ def read_vtt(path: str) -> Dict[str, np.ndarray]:
     dec = _Decoder(root, None, "raw")
     rd = root.find("Table").find("Piece").find("RowData")
     return {el.get("Name"): dec.array(el) for el in rd.findall("DataArray")}
+
+
+# ---------------------------------------------------------------- partitioned (parallel) files
+
+def piece_name(path: str, piece: int) -> str:
+    """The file name VTK's parallel XML writers give piece ``piece`` of ``path``:
+    ``<base>_<piece>.<serial extension>`` beside the summary file."""
+    base, ext = os.path.splitext(path)
+    return "%s_%d.%s" % (base, piece, {".pvtu": "vtu", ".pvtt": "vtt"}[ext])
+
+
+def _p_arrays(tag: str, d: Dict[str, np.ndarray]) -> str:
+    if not d:
+        return ""
+    out = []
+    for k, v in d.items():
+        a = np.asarray(v)
+        ncomp = a.shape[1] if a.ndim == 2 else 1
+        out.append('<PDataArray type="%s" Name="%s" NumberOfComponents="%d"/>'
+                   % (_NAMES[a.dtype.newbyteorder("<").str], k, ncomp))
+    return "<%s>%s</%s>" % (tag, "".join(out), tag)
+
+
+def write_pvtu(path: str, points, cells, point_data=None, cell_data=None, field_data=None, fmt="appended",
+               header="UInt64", compress=False):
+    """A one-piece partitioned grid: the ``.pvtu`` summary (the arrays' names and types and
+    the piece's file) and the piece ``<base>_0.vtu``, as the reference writes solutions when
+    it runs on several processes (VTKPWrite.hpp:100-137 through vtkXMLPUnstructuredGridWriter)."""
+    if not path.endswith(".pvtu"):
+        raise VtkError("%s: a partitioned grid file ends in .pvtu" % path)
+    piece = piece_name(path, 0)
+    write_vtu(piece, points, cells, point_data, cell_data, field_data, fmt=fmt, header=header, compress=compress)
+    with open(path, "w") as f:
+        f.write('<?xml version="1.0"?>\n<VTKFile type="PUnstructuredGrid" version="1.0" byte_order="LittleEndian" '
+                'header_type="%s">\n<PUnstructuredGrid GhostLevel="0">%s%s%s'
+                '<PPoints><PDataArray type="Float64" Name="Points" NumberOfComponents="3"/></PPoints>'
+                '<Piece Source="%s"/></PUnstructuredGrid>\n</VTKFile>\n'
+                % (header, _p_arrays("PFieldData", field_data), _p_arrays("PPointData", point_data),
+                   _p_arrays("PCellData", cell_data), os.path.basename(piece)))
+
+
+def write_pvtt(path: str, columns: Dict[str, np.ndarray]):
+    """A one-piece partitioned table: ``.pvtt`` summary plus the piece ``<base>_0.vtt``, the
+    layout the reference's post-processing reads (py_utils/PVTTReader.py: ``PTable/Piece``
+    ``Source`` attributes, each piece a vtkTable)."""
+    if not path.endswith(".pvtt"):
+        raise VtkError("%s: a partitioned table file ends in .pvtt" % path)
+    piece = piece_name(path, 0)
+    write_vtt(piece, columns)
+    cols = {k: np.asarray(v, np.float64) for k, v in columns.items()}
+    with open(path, "w") as f:
+        f.write('<?xml version="1.0"?>\n<VTKFile type="PTable" version="1.0" byte_order="LittleEndian">\n'
+                '<PTable>%s<Piece Source="%s"/></PTable>\n</VTKFile>\n'
+                % (_p_arrays("PRowData", cols), os.path.basename(piece)))
+
+
+def _pieces(path: str, kind: str):
+    root = ET.parse(path).getroot()
+    if root.tag != "VTKFile" or root.get("type") != kind:
+        raise VtkError("%s: not a VTK %s file" % (path, kind))
+    here = os.path.dirname(os.path.abspath(path))
+    return [os.path.join(here, el.get("Source")) for el in root.find(kind).findall("Piece")]
+
+
+def read_pvtt(path: str) -> Dict[str, np.ndarray]:
+    """The rows of every piece of a ``.pvtt`` table, in piece order (as PVTTReader
+    concatenates them); piece paths are taken relative to the summary file."""
+    parts = [read_vtt(p) for p in _pieces(path, "PTable")]
+    if not parts:
+        return {}
+    return {k: np.concatenate([np.asarray(t[k]).reshape(-1) for t in parts]) for k in parts[0]}
+
+
+def read_pvtu(path: str) -> VtuFile:
+    """The pieces of a ``.pvtu`` grid appended in piece order (points of later pieces after
+    those of earlier ones, their cells renumbered accordingly; points shared by two pieces
+    appear once per piece, as VTK's reader returns them); FieldData from the first piece."""
+    parts = [read_vtu(p) for p in _pieces(path, "PUnstructuredGrid")]
+    if not parts:
+        raise VtkError("%s has no pieces" % path)
+    base = np.cumsum([0] + [q.points.shape[0] for q in parts[:-1]])
+    cells = np.concatenate([q.cells.astype(np.int64) + b for q, b in zip(parts, base)]).astype(np.int32)
+
+    def cat(key):
+        return {k: np.concatenate([getattr(q, key)[k] for q in parts]) for k in getattr(parts[0], key)}
+
+    return VtuFile(np.concatenate([q.points for q in parts]), cells, cat("point_data"), cat("cell_data"),
+                   parts[0].field_data)
 
 
 # ---------------------------------------------------------------- jobs

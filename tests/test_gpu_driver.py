@@ -1,6 +1,7 @@
 """The TransducerModel driver on the DeviceSolver, from a job .vtu to solution and table
-files: every velocity table and the last solution equal the oracle run of the same job, bit
-for bit, with the step counts of the reference's write-time schedule."""
+files (serial names, and the partitioned .pvtu/.pvtt layout): every velocity table and the
+last solution equal the oracle run of the same job, bit for bit, with the step counts of the
+reference's write-time schedule."""
 import numpy as np
 import pytest
 
@@ -10,27 +11,37 @@ from pannde_amd import cases, rsg, transducer_model, vtkio
 pytestmark = pytest.mark.gpu
 
 
-def test_job_file_to_outputs_matches_oracle(tmp_path):
+@pytest.mark.parametrize("partitioned", [False, True])
+def test_job_file_to_outputs_matches_oracle(tmp_path, monkeypatch, partitioned):
     src = cases.al_block(seed=41, nx=11, ny=9, nz=6, nsteps=40, n_sensors=1)
     wt = np.array([0.0, 7.3, 15.5, 30.2, 38.0]) * src.dt  # uneven intervals, one at t = 0
     job = tmp_path / "job.vtu"
     vtkio.case_to_job(src, str(job), wt, fmt="appended", compress=True)
-    out = str(tmp_path / "run")
-    r = transducer_model.run(str(job), out, rsg.RSG_FP64)
+    run_dir = tmp_path / "run"
+    run_dir.mkdir()
+    monkeypatch.chdir(run_dir)  # the tables go to the working directory, as the reference writes them
+    out = str(tmp_path / "solution")
+    r = transducer_model.run(str(job), out, rsg.RSG_FP64, partitioned=partitioned)
     case, _ = vtkio.job_case(str(job))
     steps = vtkio.schedule_steps(case.dt, wt)
     assert r["steps"] == steps[-1] and steps[0] == 0
     V, S, t, tr = orc.run_oracle(case, nsteps=int(steps[-1]))
-    last = vtkio.read_vtu("%s_%d.vtu" % (out, len(wt) - 1))
+    if partitioned:
+        last = vtkio.read_pvtu("%s_%d.pvtu" % (out, len(wt) - 1))
+        assert (tmp_path / ("solution_%d_0.vtu" % (len(wt) - 1))).exists()
+        table, ext = vtkio.read_pvtt, ".pvtt"
+    else:
+        last = vtkio.read_vtu("%s_%d.vtu" % (out, len(wt) - 1))
+        table, ext = vtkio.read_vtt, ".vtt"
     for k, name in enumerate(rsg.STATE_V):
         assert np.array_equal(last.point_data[name], V[k]), name
     for k, name in enumerate(rsg.STATE_S):
         assert np.array_equal(last.cell_data[name], S[k]), name
     assert last.field_data["time"][0] == t
-    coords = vtkio.read_vtt(out + "_node_coords.vtt")
+    coords = table(str(run_dir / ("node_coords" + ext)))
     assert np.array_equal(coords["node_x"], case.x[case.sensors]) and case.sensors.shape[0] > 0
     for k, n in enumerate(steps):
-        vel = vtkio.read_vtt("%s_velocities_%d.vtt" % (out, k))
+        vel = table(str(run_dir / ("velocities_%d%s" % (k, ext))))
         ref = tr[n - 1] if n else np.zeros((case.sensors.shape[0], 3))
         for c, name in enumerate(("Vx", "Vy", "Vz")):
             assert np.array_equal(vel[name], ref[:, c]), (k, name)

@@ -4,6 +4,7 @@ big-endian inline base64, a zlib block header built field by field, a voxel cell
 job conversion (fields, transducers, timing, sensors) against the oracle's schedule."""
 import base64
 import struct
+import xml.etree.ElementTree as ET
 import zlib
 
 import numpy as np
@@ -178,3 +179,50 @@ def test_schedule_matches_the_oracle():
         dt = float(rng.uniform(1e-9, 1e-8))
         wt = np.cumsum(rng.uniform(0, 40, size=6)) * dt
         assert np.array_equal(vtkio.schedule_steps(dt, wt), np.cumsum(orc.schedule(dt, wt)))
+
+
+def test_partitioned_files(tmp_path):
+    """.pvtu/.pvtt summaries name their pieces as VTK's parallel writers do and read back whole;
+    the table summary has the layout py_utils/PVTTReader.py parses (PTable/Piece Source)."""
+    rng = np.random.default_rng(5)
+    pts, cells = two_hexes()
+    pdata = {"Vx": rng.standard_normal(pts.shape[0])}
+    cdata = {"Sxx": rng.standard_normal(2), "Syy": rng.standard_normal(2)}
+    p = tmp_path / "out_3.pvtu"
+    vtkio.write_pvtu(str(p), pts, cells, pdata, cdata, {"time": np.array([1.5e-6])})
+    assert vtkio.piece_name(str(p), 0) == str(tmp_path / "out_3_0.vtu") and (tmp_path / "out_3_0.vtu").exists()
+    root = ET.parse(str(p)).getroot()
+    assert root.get("type") == "PUnstructuredGrid"
+    assert [e.get("Name") for e in root.find("PUnstructuredGrid").find("PCellData")] == ["Sxx", "Syy"]
+    v = vtkio.read_pvtu(str(p))
+    assert np.array_equal(v.points, pts) and np.array_equal(v.cells, cells)
+    assert np.array_equal(v.point_data["Vx"], pdata["Vx"]) and np.array_equal(v.cell_data["Syy"], cdata["Syy"])
+    assert v.field_data["time"][0] == 1.5e-6
+    cols = {"Vx": rng.standard_normal(4), "Vy": rng.standard_normal(4), "Vz": rng.standard_normal(4)}
+    t = tmp_path / "velocities_3.pvtt"
+    vtkio.write_pvtt(str(t), cols)
+    pieces = [e.attrib["Source"] for e in ET.parse(str(t)).findall("PTable/Piece")]  # PVTTReader's lookup
+    assert pieces == ["velocities_3_0.vtt"]
+    assert all(np.array_equal(vtkio.read_vtt(str(tmp_path / pieces[0]))[k], v_) for k, v_ in cols.items())
+    back = vtkio.read_pvtt(str(t))
+    assert all(np.array_equal(back[k], v_) for k, v_ in cols.items())
+    with pytest.raises(vtkio.VtkError, match="pvtt"):
+        vtkio.write_pvtt(str(tmp_path / "x.vtt"), cols)
+
+
+def test_pvtu_pieces_append_in_order(tmp_path):
+    """Two pieces written by hand: points appended, the second piece's cells renumbered."""
+    pts, cells = two_hexes()
+    a, b = tmp_path / "g_0.vtu", tmp_path / "g_1.vtu"
+    vtkio.write_vtu(str(a), pts, cells, {"Vx": np.arange(pts.shape[0], dtype=np.float64)}, fmt="ascii")
+    vtkio.write_vtu(str(b), pts + 10.0, cells[:1], {"Vx": -np.arange(pts.shape[0], dtype=np.float64)}, fmt="binary")
+    (tmp_path / "g.pvtu").write_text(
+        '<?xml version="1.0"?><VTKFile type="PUnstructuredGrid" version="1.0"><PUnstructuredGrid GhostLevel="0">'
+        '<PPointData><PDataArray type="Float64" Name="Vx"/></PPointData>'
+        '<PPoints><PDataArray type="Float64" NumberOfComponents="3"/></PPoints>'
+        '<Piece Source="g_0.vtu"/><Piece Source="g_1.vtu"/></PUnstructuredGrid></VTKFile>')
+    v = vtkio.read_pvtu(str(tmp_path / "g.pvtu"))
+    n = pts.shape[0]
+    assert v.points.shape == (2 * n, 3) and np.array_equal(v.points[n:], pts + 10.0)
+    assert np.array_equal(v.cells, np.concatenate([cells, cells[:1] + n]))
+    assert np.array_equal(v.point_data["Vx"], np.concatenate([np.arange(n), -np.arange(n)]).astype(np.float64))
