@@ -406,6 +406,17 @@ def read_pvtu(path: str) -> VtuFile:
 ELASTIC = ["C%d%d" % (i + 1, j + 1) for i in range(6) for j in range(i, 6)]
 
 
+def _job_domain(v: VtuFile):
+    """Mesh, density, C11..C66 and the window depth (z-size of cell 0, Boss.hpp:530-536 and
+    :639-645) of a job file."""
+    x, y, z = (np.ascontiguousarray(v.points[:, d]) for d in range(3))
+    cells = v.cells
+    if cells.shape[0] == 0:
+        raise VtkError("the job has no cells")
+    ds = abs(z[cells[0, 0]] - z[cells[0, 6]])
+    return x, y, z, cells, v.field("density"), [v.field(n) for n in ELASTIC], ds
+
+
 def job_case(path: str, name: Optional[str] = None) -> "tuple[cases.Case, np.ndarray]":
     """A TransducerModel job (TransducerModel.cpp:160-204): mesh, density and C11..C66
     fields, transducers XD<k>/{XCenter,YCenter,ZCenter,Radius,SignalTimes,SignalValues} driving
@@ -413,11 +424,7 @@ def job_case(path: str, name: Optional[str] = None) -> "tuple[cases.Case, np.nda
     write_times.  Sensors are the nodes any transducer's window accepts
     (TransducerModel.cpp:106-130).  Returns (case, write_times)."""
     v = read_vtu(path)
-    x, y, z = (np.ascontiguousarray(v.points[:, d]) for d in range(3))
-    cells = v.cells
-    if cells.shape[0] == 0:
-        raise VtkError("the job has no cells")
-    ds = abs(z[cells[0, 0]] - z[cells[0, 6]])
+    x, y, z, cells, rho, cij, ds = _job_domain(v)
     discs = []
     nxd = int(v.value("NTransducers"))
     for k in range(nxd):
@@ -433,9 +440,39 @@ def job_case(path: str, name: Optional[str] = None) -> "tuple[cases.Case, np.nda
         sens |= (np.abs(z - d.center[2]) <= 0.75 * d.ds) & (dist < d.radius) & (dist >= 0)
     steps = schedule_steps(dt, write_times)
     case = cases.Case(name=name or os.path.splitext(os.path.basename(path))[0], x=x, y=y, z=z, cells=cells,
-                      rho=v.field("density"), cij=[v.field(n) for n in ELASTIC], dt=dt,
-                      nsteps=int(steps[-1]) if len(steps) else 0, force=discs,
+                      rho=rho, cij=cij, dt=dt, nsteps=int(steps[-1]) if len(steps) else 0, force=discs,
                       sensors=np.nonzero(sens)[0].astype(np.int32))
+    return case, write_times
+
+
+WATER_COLUMN_SIGNALS = ("SignalSigmaXX", "SignalSigmaYY", "SignalSigmaZZ")
+
+
+def water_column_case(path: str, name: Optional[str] = None) -> "tuple[cases.Case, np.ndarray]":
+    """A WaterColumnModel job (WaterColumnModel.cpp:85-180): mesh, density and C11..C66,
+    water columns WC<k>/{XCenter,YCenter,ZCenter,Radius,SignalTimes,SignalSigmaXX,
+    SignalSigmaYY,SignalSigmaZZ} (StandardNames.hpp:262-290), each an XX, YY, ZZ moment
+    excitation of the HostUTModel form with the z-size of cell 0 as window depth
+    (Boss.hpp:555-650), dt and write_times.  The discs are listed column by column, XX, YY,
+    ZZ within a column, which is the order the reference sums them in for each stress
+    component (HostUTModel.hpp:296-311).  No sensors: the reference records none.  Returns
+    (case, write_times)."""
+    v = read_vtu(path)
+    x, y, z, cells, rho, cij, ds = _job_domain(v)
+    discs = []
+    for k in range(int(v.value("NWaterColumns"))):
+        p = "WC%d/" % k
+        center = (v.value(p + "XCenter"), v.value(p + "YCenter"), v.value(p + "ZCenter"))
+        t = v.array(p + "SignalTimes")
+        for comp, sig in enumerate(WATER_COLUMN_SIGNALS):
+            discs.append(cases.Disc(comp, center, v.value(p + "Radius"), ds, t, v.array(p + sig)))
+    dt = v.value("dt")
+    write_times = v.array("write_times")
+    steps = schedule_steps(dt, write_times)
+    case = cases.Case(name=name or os.path.splitext(os.path.basename(path))[0], x=x, y=y, z=z, cells=cells,
+                      rho=rho, cij=cij, dt=dt, nsteps=int(steps[-1]) if len(steps) else 0, force=[],
+                      sensors=np.zeros(0, np.int32))
+    case.moment, case.moment_form = discs, 1
     return case, write_times
 
 
@@ -451,8 +488,7 @@ def schedule_steps(dt: float, write_times) -> np.ndarray:
     return np.array(out, dtype=np.int64)
 
 
-def case_to_job(case: cases.Case, path: str, write_times, fmt="appended", header="UInt64", compress=False):
-    """Writes a Case with z-force transducers as a TransducerModel job file."""
+def _domain_arrays(case: cases.Case):
     pdata, cdata = {}, {}
     fields = [("density", case.rho)] + list(zip(ELASTIC, case.cij))
     for name, f in fields:
@@ -462,6 +498,12 @@ def case_to_job(case: cases.Case, path: str, write_times, fmt="appended", header
             cdata[name] = f.data
         else:
             cdata[name] = np.full(case.n_cells, float(f.data[0]))
+    return pdata, cdata
+
+
+def case_to_job(case: cases.Case, path: str, write_times, fmt="appended", header="UInt64", compress=False):
+    """Writes a Case with z-force transducers as a TransducerModel job file."""
+    pdata, cdata = _domain_arrays(case)
     fd = {"NTransducers": np.array([float(len(case.force))]), "dt": np.array([case.dt]),
           "write_times": np.asarray(write_times, np.float64)}
     for k, d in enumerate(case.force):
@@ -472,5 +514,31 @@ def case_to_job(case: cases.Case, path: str, write_times, fmt="appended", header
         fd[p + "Radius"] = np.array([float(d.radius)])
         fd[p + "SignalTimes"] = np.asarray(d.signal_t, np.float64)
         fd[p + "SignalValues"] = np.asarray(d.signal_v, np.float64)
+    write_vtu(path, np.stack([case.x, case.y, case.z], 1), case.cells, pdata, cdata, fd, fmt=fmt, header=header,
+              compress=compress)
+
+
+def case_to_water_column_job(case: cases.Case, path: str, write_times, fmt="appended", header="UInt64",
+                             compress=False):
+    """Writes a Case of the HostUTModel form whose moment discs come in XX, YY, ZZ triples (one
+    per water column, sharing centre, radius and signal times) as a WaterColumnModel job."""
+    m = list(case.moment)
+    if case.moment_form != 1 or case.force or len(m) % 3:
+        raise VtkError("a water-column job needs XX, YY, ZZ moment triples of the HostUTModel form")
+    pdata, cdata = _domain_arrays(case)
+    fd = {"NWaterColumns": np.array([float(len(m) // 3)]), "dt": np.array([case.dt]),
+          "write_times": np.asarray(write_times, np.float64)}
+    for k in range(len(m) // 3):
+        col = m[3 * k:3 * k + 3]
+        if [d.component for d in col] != [0, 1, 2] or any(
+                tuple(d.center) != tuple(col[0].center) or d.radius != col[0].radius
+                or not np.array_equal(d.signal_t, col[0].signal_t) for d in col):
+            raise VtkError("water column %d: discs must be XX, YY, ZZ with one centre, radius and time table" % k)
+        p = "WC%d/" % k
+        fd[p + "XCenter"], fd[p + "YCenter"], fd[p + "ZCenter"] = (np.array([float(c)]) for c in col[0].center)
+        fd[p + "Radius"] = np.array([float(col[0].radius)])
+        fd[p + "SignalTimes"] = np.asarray(col[0].signal_t, np.float64)
+        for d, sig in zip(col, WATER_COLUMN_SIGNALS):
+            fd[p + sig] = np.asarray(d.signal_v, np.float64)
     write_vtu(path, np.stack([case.x, case.y, case.z], 1), case.cells, pdata, cdata, fd, fmt=fmt, header=header,
               compress=compress)

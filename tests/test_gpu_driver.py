@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle_rsg as orc
-from pannde_amd import cases, rsg, transducer_model, vtkio
+from pannde_amd import cases, rsg, transducer_model, vtkio, water_column_model
 
 pytestmark = pytest.mark.gpu
 
@@ -45,3 +45,28 @@ def test_job_file_to_outputs_matches_oracle(tmp_path, monkeypatch, partitioned):
         ref = tr[n - 1] if n else np.zeros((case.sensors.shape[0], 3))
         for c, name in enumerate(("Vx", "Vy", "Vz")):
             assert np.array_equal(vel[name], ref[:, c]), (k, name)
+
+
+def test_water_column_job_matches_oracle(tmp_path):
+    """WaterColumnModel counterpart: two water columns (XX, YY, ZZ moments of the HostUTModel
+    form); every written solution equals the oracle's at its step count."""
+    from test_vtkio import two_column_case
+
+    src = two_column_case(nsteps=30)
+    wt = np.array([0.0, 9.5, 21.2, 29.0]) * src.dt
+    job = tmp_path / "wc.vtu"
+    vtkio.case_to_water_column_job(src, str(job), wt, fmt="binary", compress=True)
+    out = str(tmp_path / "wc_out")
+    r = water_column_model.run(str(job), out, rsg.RSG_FP64)
+    case, _ = vtkio.water_column_case(str(job))
+    steps = vtkio.schedule_steps(case.dt, wt)
+    assert r["steps"] == steps[-1]
+    for k, n in enumerate(steps):
+        sol = vtkio.read_vtu("%s_%d.vtu" % (out, k))
+        V, S, t, _ = orc.run_oracle(case, nsteps=int(n))
+        for j, name in enumerate(rsg.STATE_V):
+            assert np.array_equal(sol.point_data[name], V[j]), (k, name)
+        for j, name in enumerate(rsg.STATE_S):
+            assert np.array_equal(sol.cell_data[name], S[j]), (k, name)
+        assert sol.field_data["time"][0] == t
+    assert np.abs(sol.cell_data["Sxx"]).max() > 0

@@ -226,3 +226,44 @@ def test_pvtu_pieces_append_in_order(tmp_path):
     assert v.points.shape == (2 * n, 3) and np.array_equal(v.points[n:], pts + 10.0)
     assert np.array_equal(v.cells, np.concatenate([cells, cells[:1] + n]))
     assert np.array_equal(v.point_data["Vx"], np.concatenate([np.arange(n), -np.arange(n)]).astype(np.float64))
+
+
+def two_column_case(seed=12, nsteps=30):
+    """A water-column block with a second column of its own centre, radius and signals."""
+    case = cases.water_column_block(seed=seed, nx=10, ny=9, nz=5, nsteps=nsteps)
+    d0 = case.moment[0]
+    t = d0.signal_t
+    c1 = (0.3 * 10 * 1.0e-4, 0.35 * 9 * 1.0e-4, d0.center[2])
+    case.moment = list(case.moment) + [cases.Disc(k, c1, 0.6 * d0.radius, d0.ds, t, s * d0.signal_v[::-1])
+                                       for k, s in enumerate((0.7, -1.3, 0.4))]
+    return case
+
+
+def test_water_column_job_round_trip(tmp_path):
+    case = two_column_case()
+    wt = np.arange(4) * 8.5 * case.dt
+    p = tmp_path / "wc.vtu"
+    vtkio.case_to_water_column_job(case, str(p), wt)
+    got, wt2 = vtkio.water_column_case(str(p))
+    assert np.array_equal(wt2, wt) and got.dt == case.dt and got.moment_form == 1 and not got.force
+    assert np.array_equal(got.cells, case.cells) and np.array_equal(got.z, case.z)
+    assert len(got.moment) == len(case.moment) == 6
+    for a, b in zip(got.moment, case.moment):
+        assert a.component == b.component and tuple(a.center) == tuple(b.center) and a.radius == b.radius
+        assert a.ds == b.ds and np.array_equal(a.signal_t, b.signal_t) and np.array_equal(a.signal_v, b.signal_v)
+    assert got.sensors.shape[0] == 0
+    bad = cases.water_column_block(seed=3)
+    bad.moment = bad.moment[:2]
+    with pytest.raises(vtkio.VtkError, match="triples"):
+        vtkio.case_to_water_column_job(bad, str(tmp_path / "bad.vtu"), wt)
+
+
+@pytest.mark.skipif(not orc.have_reference(), reason="oracle/_ref/ref_rsg not built (reference tree absent)")
+def test_two_water_columns_oracle_matches_reference():
+    """Moment terms of two columns on the same components are summed column by column for
+    each cell vertex, as HostUTModel::evalMoments does: the oracle equals the reference."""
+    case = two_column_case(nsteps=20)
+    V, S, t, tr = orc.run_oracle(case)
+    V2, S2, t2, tr2, _ = orc.run_reference(case, ranks=2)
+    assert np.abs(S).max() > 0
+    assert t == t2 and np.array_equal(V, V2) and np.array_equal(S, S2)
