@@ -267,3 +267,39 @@ def test_two_water_columns_oracle_matches_reference():
     V2, S2, t2, tr2, _ = orc.run_reference(case, ranks=2)
     assert np.abs(S).max() > 0
     assert t == t2 and np.array_equal(V, V2) and np.array_equal(S, S2)
+
+
+@pytest.mark.parametrize("kind", ["plate", "bracket", "block", "water-column"])
+def test_case_maker_jobs(tmp_path, kind):
+    """Job files from the case generators read back as the generator's case, with the case
+    makers' write-time recipe."""
+    from pannde_amd import case_maker
+
+    p = tmp_path / ("%s.vtu" % kind)
+    argv = [kind, "-o", str(p), "--seed", "7", "--nsteps", "37", "--write-interval", "10"]
+    argv += {"plate": ["--nx", "24", "--ny", "20", "--cells-per-ply", "1", "--holes", "1", "--crack-length", "4e-4",
+                       "--crack-depth", "3e-4", "--crack-angle", "30"],
+             "bracket": ["--leg", "24", "--width", "8", "--height", "6", "--crack-length", "3e-4",
+                         "--crack-depth", "1e-4", "--crack-angle", "60"],
+             "block": ["--nx", "7", "--ny", "6", "--nz", "5"],
+             "water-column": ["--nx", "7", "--ny", "6", "--nz", "5"]}[kind]
+    case_maker.main(argv)
+    read = vtkio.water_column_case if kind == "water-column" else vtkio.job_case
+    got, wt = read(str(p))
+    dt = got.dt
+    assert np.array_equal(wt, np.arange(int(np.arange(0, 37 * dt, dt).shape[0] / 10) + 1) * 10 * dt)
+    params = dict(nx=24 if kind == "plate" else 7, ny=20 if kind == "plate" else 6, nz=5,
+                  dx=1.0e-4 if kind in ("block", "water-column") else 5.0e-5, cells_per_ply=1, holes=1,
+                  renumber=False, leg=24, width=8, height=6)
+    if kind == "plate":
+        params.update(crack_length=4e-4, crack_depth=3e-4, crack_angle=30.0)
+    if kind == "bracket":
+        params.update(crack_length=3e-4, crack_depth=1e-4, crack_angle=60.0)
+    ref = case_maker.make_case(kind, 7, 37, **params)
+    assert got.dt == ref.dt and np.array_equal(got.cells, ref.cells) and np.array_equal(got.x, ref.x)
+    for a, b in zip(got.cij, ref.cij):
+        assert np.array_equal(a.data, np.broadcast_to(b.data, a.data.shape))
+    src = got.moment if kind == "water-column" else got.force
+    assert len(src) == len(ref.moment if kind == "water-column" else ref.force) > 0
+    if kind in ("plate", "bracket"):
+        assert ref.meta["n_dup"] > 0  # the crack opened
