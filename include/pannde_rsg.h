@@ -116,7 +116,9 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
 /* Kernel and layout choices of a handle (they never change results: every choice is
  * bit-identical).  A zero field takes the default shown. */
 typedef struct {
-  int32_t block_velocity;      /* workgroup size of the velocity kernel, 64..512 step 64 (256) */
+  int32_t block_velocity;      /* workgroup size of the velocity kernel, 64..512 step 64 (256;
+                                  512 in FP32 for meshes of 3 x 2^20 nodes or more when
+                                  per_thread_velocity is left at 0) */
   int32_t block_stress;        /* workgroup size of the stress kernel (256) */
   int32_t per_thread_velocity; /* nodes per thread: 1, 2 or 4 (1) */
   int32_t per_thread_stress;   /* cells per thread: 1, 2 or 4 (2 for meshes of 3 x 2^20 cells
@@ -187,8 +189,11 @@ int rsg_get_time(rsg_handle h, double* time, int64_t* steps);
  * stress  = sum_c 0.5*|S_c|^2*dxc*dyc*dzc (a norm of the stress field, no compliance). */
 int rsg_energy(rsg_handle h, double* kinetic, double* stress_norm);
 
-/* Device-kernel timing: when enabled, every velocity/stress launch is bracketed by HIP
- * events on the handle's stream; totals accumulate until read (and reset). */
+/* Device-kernel timing: with enabled = n > 0 the velocity and stress launches of every n-th
+ * step (the first of each n) are bracketed by HIP events on the handle's stream (1: every
+ * step; events between launches slow the stream, so a stride keeps the timed run close to
+ * an untimed one); 0 turns timing off.  Totals and the number of timed steps accumulate
+ * until read (and reset). */
 int rsg_set_timing(rsg_handle h, int32_t enabled);
 int rsg_get_kernel_times(rsg_handle h, double* velocity_ms, double* stress_ms,
                          int64_t* launches);

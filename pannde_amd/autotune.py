@@ -4,9 +4,10 @@ Sweeps, for each of the two kernels independently, the workgroup size {64, 128, 
 the entities per thread {1, 2, 4} and LDS staging on/off, timing each candidate with the
 library's HIP-event kernel timers on a representative job, and returns the fastest
 choice as an ``rsg_tuning`` dict.  Every candidate computes bit-identical results, so the
-choice only affects speed.  Handles of one batch must share workgroup sizes and
-per-thread counts; tune once on a representative instance and create every member of the
-batch with the result.
+choice only affects speed.  Members of a batch may differ in their tuning (each launch
+covers the members of one kernel configuration), but members of one configuration share
+launches, so tune once on a representative instance and create the batch's members with
+the result.
 """
 from __future__ import annotations
 

@@ -230,7 +230,7 @@ struct rsg_instance {
   DevBuf<double> partial;
 
   // timing
-  bool timing = false;
+  int32_t timing = 0;  // 0: off; n > 0: every n-th step's two launches are timed
   std::vector<hipEvent_t> tev;
   double t_vel_ms = 0, t_str_ms = 0;
   int64_t t_launches = 0;
@@ -947,39 +947,45 @@ int run_block(rsg_instance* h, int kb, const double* amp_dev) {
   fill_args<T>(h, va, sa);
   const size_t nch = std::max<size_t>(1, h->channels.size());
   const size_t row = h->sensors.size() * 3;
-  if (h->timing) {
-    int rc = ensure_events(h, (size_t)2 * kb + 1);
+  // timed steps get three events: before the velocity launch, between the two, after the
+  // stress launch (events between every launch slow the stream down, so a stride samples)
+  const int stride = h->timing;
+  const int nt = stride > 0 ? (kb + stride - 1) / stride : 0;
+  if (nt) {
+    int rc = ensure_events(h, (size_t)3 * nt);
     if (rc) return rc;
-    HIP_TRY(hipEventRecord(h->tev[0], h->stream));
   }
   for (int k = 0; k < kb; k++) {
+    const bool timed = stride > 0 && k % stride == 0;
+    hipEvent_t* ev = timed ? &h->tev[(size_t)3 * (k / stride)] : nullptr;
     va.amp = amp_dev + (size_t)k * nch;
     sa.amp = va.amp;
     va.trace = h->trace.p ? h->trace.p + (size_t)(h->trace_dev_steps + k) * row : nullptr;
     hipError_t e;
+    if (timed) HIP_TRY(hipEventRecord(ev[0], h->stream));
     if (sizeof(T) == 8)
       e = rsg::launch_velocity_f64(*(rsg::VelocityArgs<double>*)&va, h->block_v, h->stream);
     else
       e = rsg::launch_velocity_f32(*(rsg::VelocityArgs<float>*)&va, h->block_v, h->stream);
     if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "velocity launch: %s", hipGetErrorString(e));
-    if (h->timing) HIP_TRY(hipEventRecord(h->tev[2 * k + 1], h->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev[1], h->stream));
     if (sizeof(T) == 8)
       e = rsg::launch_stress_f64(*(rsg::StressArgs<double>*)&sa, h->block_s, h->stream);
     else
       e = rsg::launch_stress_f32(*(rsg::StressArgs<float>*)&sa, h->block_s, h->stream);
     if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "stress launch: %s", hipGetErrorString(e));
-    if (h->timing) HIP_TRY(hipEventRecord(h->tev[2 * k + 2], h->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev[2], h->stream));
   }
-  if (h->timing) {
-    HIP_TRY(hipEventSynchronize(h->tev[2 * kb]));
-    for (int k = 0; k < kb; k++) {
+  if (nt) {
+    HIP_TRY(hipEventSynchronize(h->tev[(size_t)3 * nt - 1]));
+    for (int q = 0; q < nt; q++) {
       float a = 0, b = 0;
-      HIP_TRY(hipEventElapsedTime(&a, h->tev[2 * k], h->tev[2 * k + 1]));
-      HIP_TRY(hipEventElapsedTime(&b, h->tev[2 * k + 1], h->tev[2 * k + 2]));
+      HIP_TRY(hipEventElapsedTime(&a, h->tev[(size_t)3 * q], h->tev[(size_t)3 * q + 1]));
+      HIP_TRY(hipEventElapsedTime(&b, h->tev[(size_t)3 * q + 1], h->tev[(size_t)3 * q + 2]));
       h->t_vel_ms += a;
       h->t_str_ms += b;
     }
-    h->t_launches += kb;
+    h->t_launches += nt;
   }
   h->trace_dev_steps += row ? kb : 0;
   return RSG_OK;
@@ -1077,6 +1083,12 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   // streaming operands are requested one cell ahead: -9 % stress time on a 3.4 M-cell plate),
   // one below (measured faster on 32^3..128^3 instances), unless the caller chose
   if (!tuning || tuning->per_thread_stress == 0) h->cpt_s = nc >= (3 << 20) ? 2 : 1;
+  // 512-node velocity workgroups in FP32 on meshes of 3 Mi nodes or more (-5 % velocity time
+  // on the 3.5 M-node plate; FP64 is faster at 256), unless the caller chose the velocity
+  // workgroup or its nodes per thread
+  if ((!tuning || (tuning->block_velocity == 0 && tuning->per_thread_velocity == 0)) && precision == RSG_FP32 &&
+      nn >= (3 << 20))
+    h->block_v = 512;
   for (int blk : {h->block_v, h->block_s})
     if (blk < 64 || blk > 512 || blk % 64 != 0)
       return fail(RSG_ERR_ARG, "workgroup size %d: must be a multiple of 64 in [64, 512]", blk);
@@ -1634,7 +1646,8 @@ int rsg_energy(rsg_handle h, double* kinetic, double* stress_norm) {
 
 int rsg_set_timing(rsg_handle h, int32_t enabled) {
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
-  h->timing = enabled != 0;
+  if (enabled < 0) return fail(RSG_ERR_ARG, "timing stride %d < 0", enabled);
+  h->timing = enabled;
   return RSG_OK;
 }
 

@@ -330,8 +330,10 @@ class DeviceSolver:
         _check(lib().rsg_energy(self._h, ctypes.byref(k), ctypes.byref(s)))
         return k.value, s.value
 
-    def set_timing(self, on=True):
-        _check(lib().rsg_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, on=True, stride=1):
+        """Times the launches of every ``stride``-th step with HIP events (off with on=False);
+        kernel_times() returns the totals and the number of timed steps."""
+        _check(lib().rsg_set_timing(self._h, int(stride) if on else 0))
 
     def kernel_times(self):
         v, s, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int64(0)

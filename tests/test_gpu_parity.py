@@ -179,6 +179,26 @@ def test_split_solve_calls_equal_one_call():
     assert np.array_equal(V, V1) and np.array_equal(S, S1) and np.array_equal(tr, tr1)
 
 
+@pytest.mark.parametrize("stride", [1, 4])
+def test_kernel_timing_counts_sampled_steps_and_leaves_results(stride):
+    """Timed runs (every step or every 4th) count the timed steps per call of up to 256
+    steps and compute the same results as untimed ones."""
+    case = cases.al_block(seed=5, nx=10, ny=9, nz=6, nsteps=300, n_sensors=2)
+    s = rsg.DeviceSolver.from_case(case)
+    s.set_timing(True, stride=stride)
+    s.solve(10)
+    s.solve(290)  # one block of 256 steps and one of 34
+    vms, sms, n = s.kernel_times()
+    assert n == sum(-(-k // stride) for k in (10, 256, 34)) and vms > 0 and sms > 0
+    with pytest.raises(rsg.RSGError, match="stride"):
+        s.set_timing(True, stride=-1)
+    s.set_timing(False)
+    V, S = s.get_state()
+    s.close()
+    V1, S1, _, _ = run_device(case)
+    assert np.array_equal(V, V1) and np.array_equal(S, S1)
+
+
 def test_caller_supplied_amplitudes_equal_library_signal():
     case = cases.al_block(seed=4, nx=10, ny=9, nz=5, nsteps=60, n_sensors=3)
     d = case.force[0]
