@@ -8,6 +8,7 @@ without the crack reaches it (a property of the scheme, not of this code)."""
 import math
 
 import numpy as np
+import pytest
 
 from oracle import oracle_rsg as orc
 from pannde_amd import cases
@@ -109,3 +110,50 @@ def test_bracket_shape_and_seeded_crack():
     assert 0.5e-3 <= cr.length <= 5.0e-3 and 0.0 <= cr.angle_deg < 180.0
     again = cases.ti_bracket(seed=7, leg=40, width=10, height=8)
     assert np.array_equal(c.cells, again.cells) and c.meta["n_dup"] == again.meta["n_dup"]
+
+
+# SHA-256 prefixes of every generated array (coordinates, connectivity, sensors, density, the
+# 21 C fields, dt and the discs) for small instances of each generator: the bench workloads,
+# the served stream and the fixtures depend on them, so a change to a generator's output has
+# to be deliberate.
+GENERATOR_DIGESTS = {
+    "al_block": "89229d0c3af8db8b",
+    "cfrp_plate_rcm": "d845fedc71af1ebc",
+    "cfrp_plate_crack": "a44e81df45f65206",
+    "ti_bracket": "651964b72ca358bf",
+    "water_column": "3de32a50c3745158",
+    "batch_instance": "a28373122e210843",
+    "served_request": "414c16a02607571f",
+}
+
+
+def _digest(c):
+    import hashlib
+
+    h = hashlib.sha256()
+    for a in (c.x, c.y, c.z, c.cells, c.sensors, c.rho.data, *[f.data for f in c.cij]):
+        h.update(np.ascontiguousarray(a).tobytes())
+    h.update(np.float64(c.dt).tobytes())
+    for d in list(c.force) + list(c.moment):
+        h.update(np.array([d.component, *d.center, d.radius, d.ds], np.float64).tobytes())
+        h.update(np.ascontiguousarray(d.signal_t).tobytes())
+        h.update(np.ascontiguousarray(d.signal_v).tobytes())
+    return h.hexdigest()[:16]
+
+
+@pytest.mark.parametrize("name", sorted(GENERATOR_DIGESTS))
+def test_generators_are_stable(name):
+    from pannde_amd import serve
+
+    make = {
+        "al_block": lambda: cases.al_block(seed=3, nx=9, ny=7, nz=5),
+        "cfrp_plate_rcm": lambda: cases.cfrp_plate(seed=7, nx=40, ny=36, cells_per_ply=2),
+        "cfrp_plate_crack": lambda: cases.cfrp_plate(
+            seed=8, nx=64, ny=48, cells_per_ply=2, renumber=False,
+            crack=cases.Crack((1.5e-3, 1.2e-3, 32 * 5e-5), 30.0, 1e-3, 5e-4)),
+        "ti_bracket": lambda: cases.ti_bracket(seed=5, leg=128, width=24, height=24),
+        "water_column": lambda: cases.water_column_block(seed=3),
+        "batch_instance": lambda: cases.batch_instances(seed=4, count=2)[1],
+        "served_request": lambda: serve.build_case(serve.request_stream(seed=9, n=3, rate=4.0)[1]),
+    }[name]
+    assert _digest(make()) == GENERATOR_DIGESTS[name]
