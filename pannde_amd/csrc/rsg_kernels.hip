@@ -177,11 +177,19 @@ __device__ __forceinline__ uint32_t resolve_key(const A& a, uint32_t key, int& s
   sp = -1;
   return key;
 }
-template <typename T>
-__device__ __forceinline__ void axis_spacings(const T* __restrict__ axis, uint32_t k, T& sx, T& sy, T& sz) {
+template <typename P, typename T>
+__device__ __forceinline__ void axis_spacings(P axis, uint32_t k, T& sx, T& sy, T& sz) {
   sx = axis[(k >> 14) & (kAxisClasses - 1)];
   sy = axis[kAxisClasses + ((k >> 7) & (kAxisClasses - 1))];
   sz = axis[2 * kAxisClasses + (k & (kAxisClasses - 1))];
+}
+
+// The class tables are written only by the host, so a wavefront-uniform read of them may go
+// through the scalar cache: a pointer in the constant address space lets the compiler use
+// scalar loads (reads only; nothing is ever stored through this path).
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* p) {
+  return (const __attribute__((address_space(4))) T*)p;
 }
 
 // 8 x int32 of one entity as two 16-byte loads; read once per step, so non-temporal.
@@ -299,8 +307,14 @@ __device__ __forceinline__ void velocity_apply(const VelocityArgs<T>& a, int n, 
   int sp = -1;
   if (keyed<KM>(a)) {
     const uint32_t k = resolve_key(a, in.key, sp);
-    idt = a.cls[k >> kMatShift];
-    axis_spacings(a.axis, k, sx, sy, sz);
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
+    if (__all(k == k0)) {  // one class for the whole wavefront: scalar loads
+      idt = as_const(a.cls)[k0 >> kMatShift];
+      axis_spacings(as_const(a.axis), k0, sx, sy, sz);
+    } else {
+      idt = a.cls[k >> kMatShift];
+      axis_spacings(a.axis, k, sx, sy, sz);
+    }
   } else {
     idt = in.idt;
     sx = in.sx;
@@ -468,11 +482,19 @@ __device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrI
   int sp = -1;
   if (keyed<KM>(a)) {
     const uint32_t k = resolve_key(a, in.key, sp);
-    T row[kMatRow];
-    load_row(a.cls + (size_t)(k >> kMatShift) * kMatRow, row);
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
+    if (__all(k == k0)) {  // one class for the whole wavefront: scalar loads
+      const auto row = as_const(a.cls + (size_t)(k0 >> kMatShift) * kMatRow);
 #pragma unroll
-    for (int j = 0; j < 21; j++) cc[j] = row[j];
-    axis_spacings(a.axis, k, sx, sy, sz);
+      for (int j = 0; j < 21; j++) cc[j] = row[j];
+      axis_spacings(as_const(a.axis), k0, sx, sy, sz);
+    } else {
+      T row[kMatRow];
+      load_row(a.cls + (size_t)(k >> kMatShift) * kMatRow, row);
+#pragma unroll
+      for (int j = 0; j < 21; j++) cc[j] = row[j];
+      axis_spacings(a.axis, k, sx, sy, sz);
+    }
   } else {
 #pragma unroll
     for (int j = 0; j < 21; j++) cc[j] = in.cc[j];
