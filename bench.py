@@ -207,7 +207,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: the DeviceSolver holds the whole mesh on one GPU; multi-GPU runs are not part of "
-              "this build (requested --gpus %d)" % args.gpus, file=sys.stderr)
+              "this benchmark (requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from pannde_amd import cases, rsg

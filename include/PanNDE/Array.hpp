@@ -1,7 +1,7 @@
 // PanNDE::Array / DataBundle / ArrayFactory: the container interface the RSG path reads
 // node coordinates, cell vertex lists and signal tables through.  Same names and
 // signatures as the reference's modules/PanNDE/include/Array.hpp:67-310, so code written
-// against one compiles against the other; written for this build.
+// against one compiles against the other; the code is this library's own.
 #pragma once
 
 #include <cstdint>

@@ -1,5 +1,5 @@
 // PanNDE::Field / FieldBundle / FieldFactory (reference modules/PanNDE/include/Field.hpp:73-349;
-// same names and signatures, written for this build).  A NODE field is read at a cell as
+// same names and signatures; the code is this library's own).  A NODE field is read at a cell as
 // the mean of its 8 vertices; a CELL or CONSTANT field is read at a node as the sum over
 // the present connected cells divided by 8.
 #pragma once

@@ -1,5 +1,5 @@
 // PanNDE::Model: a time-stepping solver (reference modules/PanNDE/include/Model.hpp:77-106;
-// same signatures, written for this build).  solve(n) advances n steps and returns the
+// same signatures; the code is this library's own).  solve(n) advances n steps and returns the
 // simulation time; getStates() returns the live solution bundle.
 #pragma once
 

@@ -1,6 +1,6 @@
 // PanNDE::MultiVariate: a source f(x, y, z, t) with an optional support test checkAt
-// (reference modules/PanNDE/include/MultiVariate.hpp:77-141; same signatures, written for
-// this build).  The default checkAt accepts every point.
+// (reference modules/PanNDE/include/MultiVariate.hpp:77-141; same signatures; the code is
+// this library's own).  The default checkAt accepts every point.
 #pragma once
 
 #include <memory>

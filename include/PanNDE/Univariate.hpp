@@ -1,5 +1,5 @@
 // PanNDE::Univariate: a function of time (reference modules/PanNDE/include/Univariate.hpp:69-81;
-// same signature, written for this build).
+// same signature; the code is this library's own).
 #pragma once
 
 namespace PanNDE {

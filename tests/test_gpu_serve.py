@@ -1,6 +1,6 @@
 """Served stream on the GPU: requests that arrive while others run are admitted as the
 budget frees up, folded into the running batch at K-step boundaries, and each result is
-bit-identical to the same request solved alone; the FP64 spot checks judge the FP32
+bit-identical to the same request solved alone; the FP64 spot checks hold the FP32
 results within 1e-4 relative L2 per sensor trace."""
 import numpy as np
 import pytest
