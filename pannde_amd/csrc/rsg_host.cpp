@@ -2069,7 +2069,7 @@ int batch_block(rsg_batch_s* b, int l, int32_t* unfinished) {
           const rsg_instance* h = b->slots[e].h;
           const int32_t nb = pass == 0 ? (h->nn + cv - 1) / cv : (h->nc + cs - 1) / cs;
           for (int32_t q = 0; q < nb; q++) {
-            natural.push_back((int32_t)e);
+            natural.push_back(rsg::batch_item_word((int32_t)e, kmax[e]));
             natural.push_back(q);
           }
         }
@@ -2180,6 +2180,8 @@ int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot) {
       break;
     }
   if (id < 0) {
+    if ((int64_t)b->slots.size() >= rsg::kMaxBatchSlots)
+      return fail(RSG_ERR_ARG, "a batch holds at most %d instances", rsg::kMaxBatchSlots);
     id = (int32_t)b->slots.size();
     b->slots.emplace_back();
   }

@@ -32,6 +32,13 @@ constexpr int kMatRow = 24;        // cell material row: the 21 C as the C array
 constexpr uint32_t kNoLink = 0x3FF;  // packed 10-bit link: no neighbour
 constexpr int kMaxPackedSet = 1022;  // largest staged set whose local indices fit 10 bits
 
+// First word of a batch work item: the instance slot (18 bits) and the number of steps of the
+// current K-step block it takes (kmax <= 4096, bits 18..30, so the word stays non-negative);
+// -1 marks an empty item.
+constexpr int kItemSlotBits = 18;
+constexpr int32_t kMaxBatchSlots = 1 << kItemSlotBits;
+inline int32_t batch_item_word(int32_t slot, int32_t kmax) { return slot | (kmax << kItemSlotBits); }
+
 inline uint32_t make_key(int mat, int ix, int iy, int iz) {
   return ((uint32_t)mat << kMatShift) | ((uint32_t)ix << 14) | ((uint32_t)iy << 7) | (uint32_t)iz;
 }

@@ -614,9 +614,10 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_kernel(S
 template <typename T, bool LDS, int U, int CPT, int KM>
 __device__ __forceinline__ void velocity_item(const BatchDesc<T>* __restrict__ desc, int2 it, int k,
                                               unsigned char* smem) {
-  if (it.x < 0) return;
-  const BatchDesc<T>* d = desc + it.x;
-  if (k >= d->kmax) return;
+  // the item carries the instance's step count for this block, so the branch needs no
+  // descriptor load and the descriptor is read in one round trip after it
+  if (it.x < 0 || k >= (it.x >> kItemSlotBits)) return;
+  const BatchDesc<T>* d = desc + (it.x & (kMaxBatchSlots - 1));
   VelocityArgs<T> a = d->v;
   a.amp = d->amp_base + (size_t)k * d->nch;
   a.trace = d->trace_base ? d->trace_base + (size_t)(d->trace_row0 + k) * d->row : nullptr;
@@ -626,9 +627,8 @@ __device__ __forceinline__ void velocity_item(const BatchDesc<T>* __restrict__ d
 template <typename T, bool LDS, int U, int CPT, int KM>
 __device__ __forceinline__ void stress_item(const BatchDesc<T>* __restrict__ desc, int2 it, int k,
                                             unsigned char* smem) {
-  if (it.x < 0) return;
-  const BatchDesc<T>* d = desc + it.x;
-  if (k >= d->kmax) return;
+  if (it.x < 0 || k >= (it.x >> kItemSlotBits)) return;
+  const BatchDesc<T>* d = desc + (it.x & (kMaxBatchSlots - 1));
   StressArgs<T> a = d->s;
   a.amp = d->amp_base + (size_t)k * d->nch;
   stress_body<T, LDS, U, CPT, KM>(a, it.y, smem);
