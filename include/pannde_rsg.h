@@ -237,8 +237,9 @@ typedef struct rsg_batch_s* rsg_batch;
 int rsg_batch_create(int32_t precision, int32_t steps_per_block, int32_t grid, int32_t use_graph,
                      rsg_batch* out);
 
-/* Adds a handle that will take nsteps more steps; *slot receives its slot id.  It joins at
- * the next K-step block; blocks already enqueued keep running.  Members may differ in
+/* Adds a handle that will take nsteps more steps; *slot receives its slot id (a batch holds
+ * at most 2^18 instances at once).  It joins at the next K-step block; blocks already
+ * enqueued keep running.  Members may differ in
  * their tuning: each launch covers the members of one kernel configuration. */
 int rsg_batch_add(rsg_batch b, rsg_handle h, int64_t nsteps, int32_t* slot);
 
