@@ -33,11 +33,28 @@ def _run(cmd, verbose):
 def build(verbose=False):
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
-    objs = []
-    for src, flags in (("rsg_host.cpp", HOST_FLAGS), ("rsg_kernels.hip", DEVICE_FLAGS)):
-        obj = os.path.join(bdir, src + ".o")
-        _run([HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+    # the host runtime and the four parts of the kernel library (FP64 / FP32, single-instance /
+    # batched; see RSG_PART in rsg_kernels.hip) compile in parallel
+    jobs = [([HIPCC] + HOST_FLAGS + ["-c", os.path.join(CSRC, "rsg_host.cpp"), "-o",
+                                     os.path.join(bdir, "rsg_host.cpp.o")])]
+    for part in (1, 2, 3, 4):
+        jobs.append([HIPCC] + DEVICE_FLAGS + ["-DRSG_PART=%d" % part, "-c", os.path.join(CSRC, "rsg_kernels.hip"),
+                                              "-o", os.path.join(bdir, "rsg_kernels_part%d.o" % part)])
+    objs = [cmd[-1] for cmd in jobs]
+    procs = []
+    for cmd in jobs:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = []
+    for cmd, pr in procs:
+        out = "\n".join(l for l in pr.communicate()[0].splitlines() if "packed-fp32-ops" not in l)
+        if pr.returncode != 0:
+            failed.append("build failed: %s\n%s" % (" ".join(cmd), out))
+        elif verbose and out.strip():
+            print(out)
+    if failed:
+        raise RuntimeError("\n".join(failed))
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs + ["-o", tmp], verbose)
     os.replace(tmp, OUT)

@@ -762,18 +762,40 @@ hipError_t launch_stress(const StressArgs<T>& a, int block, hipStream_t s) {
 
 }  // namespace
 
+// The exported launchers, and with them the kernel instances, are split over four
+// translation units that the build compiles in parallel: RSG_PART = 1 FP64 single-instance,
+// 2 FP32 single-instance, 3 FP64 batched, 4 FP32 batched (undefined: all of them).
+#ifndef RSG_PART
+#define RSG_PART 0
+#endif
+
+#if RSG_PART == 0 || RSG_PART == 1
 hipError_t launch_velocity_f64(const VelocityArgs<double>& a, int block, hipStream_t s) {
   return launch_velocity<double>(a, block, s);
-}
-hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStream_t s) {
-  return launch_velocity<float>(a, block, s);
 }
 hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t s) {
   return launch_stress<double>(a, block, s);
 }
+hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(energy_kernel<double>, dim3(nblocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+#endif
+
+#if RSG_PART == 0 || RSG_PART == 2
+hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStream_t s) {
+  return launch_velocity<float>(a, block, s);
+}
 hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t s) {
   return launch_stress<float>(a, block, s);
 }
+hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(energy_kernel<float>, dim3(nblocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+#endif
+
+#if RSG_PART == 0 || RSG_PART >= 3
 template <typename T>
 hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* table, int n, int k, int grid, int block,
                         bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
@@ -808,23 +830,22 @@ hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* table, int n, i
   });
 }
 
+#endif
+
+#if RSG_PART == 0 || RSG_PART == 3
 hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* table, int n, int k, int grid, int block,
                             bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node) {
   return launch_batch<double>(d, table, n, k, grid, block, lds, max_u, cpt, km, velocity, s, graph, node);
 }
+#endif
+
+#if RSG_PART == 0 || RSG_PART == 4
 hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* table, int n, int k, int grid, int block,
                             bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node) {
   return launch_batch<float>(d, table, n, k, grid, block, lds, max_u, cpt, km, velocity, s, graph, node);
 }
-hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(energy_kernel<double>, dim3(nblocks), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(energy_kernel<float>, dim3(nblocks), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
+#endif
 
 }  // namespace rsg
