@@ -1083,12 +1083,15 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   // streaming operands are requested one cell ahead: -9 % stress time on a 3.4 M-cell plate),
   // one below (measured faster on 32^3..128^3 instances), unless the caller chose
   if (!tuning || tuning->per_thread_stress == 0) h->cpt_s = nc >= (3 << 20) ? 2 : 1;
-  // 512-node velocity workgroups in FP32 on meshes of 3 Mi nodes or more (-5 % velocity time
-  // on the 3.5 M-node plate; FP64 is faster at 256), unless the caller chose the velocity
-  // workgroup or its nodes per thread
-  if ((!tuning || (tuning->block_velocity == 0 && tuning->per_thread_velocity == 0)) && precision == RSG_FP32 &&
-      nn >= (3 << 20))
+  // 512-node velocity workgroups on meshes of 3 Mi nodes or more (-5 % velocity time in FP32
+  // and -4 % in FP64 on the 3.5 M-node plate; 256 again if their staged sets do not fit in
+  // LDS or in packed links), unless the caller chose the velocity workgroup or its nodes
+  // per thread
+  bool auto_block_v = false;
+  if ((!tuning || (tuning->block_velocity == 0 && tuning->per_thread_velocity == 0)) && nn >= (3 << 20)) {
     h->block_v = 512;
+    auto_block_v = true;
+  }
   for (int blk : {h->block_v, h->block_s})
     if (blk < 64 || blk > 512 || blk % 64 != 0)
       return fail(RSG_ERR_ARG, "workgroup size %d: must be a multiple of 64 in [64, 512]", blk);
@@ -1268,8 +1271,16 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     h->ncell = (int32_t*)p;
     LdsPlan plan;
     bool packed = false;
-    if (h->lds_v && build_lds_plan(conn, nn, nc, h->block_v * h->cpt_v, plan) &&
-        (size_t)6 * plan.max_u * h->elt() <= lds_limit) {
+    auto plan_fits = [&]() {
+      return build_lds_plan(conn, nn, nc, h->block_v * h->cpt_v, plan) &&
+             rsg::velocity_lds_bytes(plan.max_u, h->elt()) <= lds_limit;
+    };
+    bool fits = h->lds_v && plan_fits();
+    if (h->lds_v && auto_block_v && (!fits || (h->n_nmat >= 0 && plan.max_u > rsg::kMaxPackedSet))) {
+      h->block_v = 256;  // the default size's sets are too large: back to 256-node workgroups
+      fits = plan_fits();
+    }
+    if (fits) {
       if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
       h->v_uoff = (int32_t*)p;
       if ((rc = upload(h.get(), &p, plan.ulist))) return rc;

@@ -32,6 +32,10 @@ constexpr int kMatRow = 24;        // cell material row: the 21 C as the C array
 constexpr uint32_t kNoLink = 0x3FF;  // packed 10-bit link: no neighbour
 constexpr int kMaxPackedSet = 1022;  // largest staged set whose local indices fit 10 bits
 
+// LDS of a velocity workgroup: 6 stress rows of max_u + 1 entries, the last one a zero that
+// links to missing cells read.
+inline size_t velocity_lds_bytes(int max_u, size_t elt) { return (size_t)6 * (max_u + 1) * elt; }
+
 // First word of a batch work item: the instance slot (18 bits) and the number of steps of the
 // current K-step block it takes (kmax <= 4096, bits 18..30, so the word stays non-negative);
 // -1 marks an empty item.
@@ -65,7 +69,7 @@ struct VelocityArgs {
   const int32_t* u_off;    // [nblocks+1]
   const int32_t* ulist;    // cells staged by each block
   const uint16_t* lconn;   // [n][8] block-local cell index by slot, 0xFFFF = none
-  int32_t max_u;           // largest unique set of a block (LDS row length)
+  int32_t max_u;           // largest unique set of a block (LDS rows hold max_u + 1)
   // special nodes (sources and/or sensors)
   const T* sp_idt;         // [nsp] real dt/rho_n
   const int32_t* sp_off;   // [nsp][4] term ranges for directions 0,1,2

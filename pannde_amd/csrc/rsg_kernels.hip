@@ -112,11 +112,13 @@ __device__ __forceinline__ float diffsum(const float (&f)[8]) {
 __device__ __forceinline__ double dscale(double v, double sp) { return v * 0.25 / sp; }
 __device__ __forceinline__ float dscale(float v, float inv) { return v * inv; }
 
-template <typename T>
+// Cells around a node: a missing cell (-1) contributes zero.  LDS-staged gathers need no
+// test, because their missing links point at a staged entry that holds zero.
+template <bool DIRECT, typename T>
 __device__ __forceinline__ void load8_cells(const T* __restrict__ f, const int (&c)[8],
                                             T (&out)[8]) {
 #pragma unroll
-  for (int k = 0; k < 8; k++) out[k] = (c[k] >= 0) ? f[c[k]] : T(0);
+  for (int k = 0; k < 8; k++) out[k] = DIRECT ? ((c[k] >= 0) ? f[c[k]] : T(0)) : f[c[k]];
 }
 template <typename T>
 __device__ __forceinline__ void load8_nodes(const T* __restrict__ f, const int (&c)[8],
@@ -201,38 +203,41 @@ __device__ __forceinline__ void load_conn(const int32_t* __restrict__ conn, int 
   c[4] = hi.x; c[5] = hi.y; c[6] = hi.z; c[7] = hi.w;
 }
 
-// 8 x uint16 block-local indices of one entity as one 16-byte load (0xFFFF = none).
-__device__ __forceinline__ void load_lconn(const uint16_t* __restrict__ lconn, int i, int (&c)[8]) {
+// 8 x uint16 block-local indices of one entity as one 16-byte load (0xFFFF = none, returned
+// as `none`).
+__device__ __forceinline__ void load_lconn(const uint16_t* __restrict__ lconn, int i, int (&c)[8], int none) {
   const v4i w = STREAM_LOAD(reinterpret_cast<const v4i*>(lconn) + (size_t)i);
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const uint32_t u = (uint32_t)w[k];
     const int lo = (int)(u & 0xFFFFu), hi = (int)(u >> 16);
-    c[2 * k] = lo == 0xFFFF ? -1 : lo;
-    c[2 * k + 1] = hi == 0xFFFF ? -1 : hi;
+    c[2 * k] = lo == 0xFFFF ? none : lo;
+    c[2 * k + 1] = hi == 0xFFFF ? none : hi;
   }
 }
 
-// 8 block-local indices (10 bits each, kNoLink = none) and the key of one entity, one
-// 16-byte load.
-__device__ __forceinline__ uint32_t load_lkey(const uint32_t* __restrict__ lkey, int i, int (&c)[8]) {
+// 8 block-local indices (10 bits each, kNoLink = none, returned as `none`) and the key of one
+// entity, one 16-byte load.
+__device__ __forceinline__ uint32_t load_lkey(const uint32_t* __restrict__ lkey, int i, int (&c)[8], int none) {
   const v4i w = STREAM_LOAD(reinterpret_cast<const v4i*>(lkey) + (size_t)i);
   const uint32_t w0 = (uint32_t)w.x, w1 = (uint32_t)w.y, w2 = (uint32_t)w.z;
   const uint32_t l[8] = {w0 & kNoLink, (w0 >> 10) & kNoLink, (w0 >> 20) & kNoLink, w1 & kNoLink,
                          (w1 >> 10) & kNoLink, (w1 >> 20) & kNoLink, w2 & kNoLink, (w2 >> 10) & kNoLink};
 #pragma unroll
-  for (int k = 0; k < 8; k++) c[k] = l[k] == kNoLink ? -1 : (int)l[k];
+  for (int k = 0; k < 8; k++) c[k] = l[k] == kNoLink ? none : (int)l[k];
   return (uint32_t)w.w;
 }
 
 // Stages NCOMP fields of the block's unique gather set (ulist[u_off[b]..u_off[b+1]))
-// into LDS as [NCOMP][max_u], each value read from HBM once per block.  With U > 0 every
+// into LDS as [NCOMP][ld], each value read from HBM once per block.  With U > 0 every
 // thread owns at most U entries and all their loads are issued before any wait (two
 // dependent round trips in all: the list, then the values); U = 0 is the general loop.
+// zero >= 0 also stores 0 at entry `zero` of every row (the target of missing links; it
+// lies past every block's set).
 template <typename T, int NCOMP, int U>
-__device__ __forceinline__ void stage(T* lds, int max_u, const int32_t* __restrict__ u_off,
+__device__ __forceinline__ void stage(T* lds, int ld, const int32_t* __restrict__ u_off,
                                       const int32_t* __restrict__ ulist, int b,
-                                      const T* const (&src)[NCOMP]) {
+                                      const T* const (&src)[NCOMP], int zero) {
   const int u0 = u_off[b], nu = u_off[b + 1] - u0;
   if constexpr (U > 0) {
     int g[U];
@@ -251,15 +256,16 @@ __device__ __forceinline__ void stage(T* lds, int max_u, const int32_t* __restri
       const int i = threadIdx.x + u * blockDim.x;
       if (i < nu)
 #pragma unroll
-        for (int k = 0; k < NCOMP; k++) lds[k * max_u + i] = v[u][k];
+        for (int k = 0; k < NCOMP; k++) lds[k * ld + i] = v[u][k];
     }
   } else {
     for (int i = threadIdx.x; i < nu; i += blockDim.x) {
       const int g = STREAM_LOAD(ulist + u0 + i);
 #pragma unroll
-      for (int k = 0; k < NCOMP; k++) lds[k * max_u + i] = src[k][g];
+      for (int k = 0; k < NCOMP; k++) lds[k * ld + i] = src[k][g];
     }
   }
+  if (zero >= 0 && (int)threadIdx.x < NCOMP) lds[threadIdx.x * ld + zero] = T(0);
   __syncthreads();
 }
 
@@ -275,9 +281,9 @@ template <typename T, bool LDS, int KM>
 __device__ __forceinline__ void velocity_load(const VelocityArgs<T>& a, int n, VelIn<T>& in) {
   if constexpr (LDS) {
     if (packed<KM>(a))
-      in.key = load_lkey(a.lkey, n, in.c);
+      in.key = load_lkey(a.lkey, n, in.c, a.max_u);  // missing cells -> the zero entry
     else
-      load_lconn(a.lconn, n, in.c);
+      load_lconn(a.lconn, n, in.c, a.max_u);
   } else {
     load_conn(a.ncell, n, in.c);
   }
@@ -297,7 +303,7 @@ __device__ __forceinline__ void velocity_load(const VelocityArgs<T>& a, int n, V
 }
 
 // V += dt/rho * (div S + f) at node n, S gathered from S0..S5 (LDS or HBM).
-template <typename T, int KM>
+template <typename T, bool LDS, int KM>
 __device__ __forceinline__ void velocity_apply(const VelocityArgs<T>& a, int n, const VelIn<T>& in,
                                                const T* S0, const T* S1, const T* S2, const T* S3,
                                                const T* S4, const T* S5) {
@@ -326,20 +332,20 @@ __device__ __forceinline__ void velocity_apply(const VelocityArgs<T>& a, int n, 
     }
   }
   T f[8];
-  load8_cells(S0, in.c, f);
+  load8_cells<!LDS>(S0, in.c, f);
   const T dx_sxx = dscale(diffsum<0>(f), sx);
-  load8_cells(S5, in.c, f);
+  load8_cells<!LDS>(S5, in.c, f);
   const T dx_sxy = dscale(diffsum<0>(f), sx);
   const T dy_sxy = dscale(diffsum<1>(f), sy);
-  load8_cells(S4, in.c, f);
+  load8_cells<!LDS>(S4, in.c, f);
   const T dx_sxz = dscale(diffsum<0>(f), sx);
   const T dz_sxz = dscale(diffsum<2>(f), sz);
-  load8_cells(S1, in.c, f);
+  load8_cells<!LDS>(S1, in.c, f);
   const T dy_syy = dscale(diffsum<1>(f), sy);
-  load8_cells(S3, in.c, f);
+  load8_cells<!LDS>(S3, in.c, f);
   const T dy_syz = dscale(diffsum<1>(f), sy);
   const T dz_syz = dscale(diffsum<2>(f), sz);
-  load8_cells(S2, in.c, f);
+  load8_cells<!LDS>(S2, in.c, f);
   const T dz_szz = dscale(diffsum<2>(f), sz);
 
   T r0 = (dx_sxx + dy_sxy) + dz_sxz;
@@ -395,16 +401,18 @@ __device__ __forceinline__ void velocity_body(const VelocityArgs<T>& a, int b, u
   if constexpr (LDS) {
     T* lds = reinterpret_cast<T*>(smem);
     const T* const src[6] = {a.S0, a.S1, a.S2, a.S3, a.S4, a.S5};
-    stage<T, 6, U>(lds, a.max_u, a.u_off, a.ulist, b, src);
+    // rows of max_u + 1 entries: entry max_u is the zero that missing cells read
+    const int ld = a.max_u + 1;
+    stage<T, 6, U>(lds, ld, a.u_off, a.ulist, b, src, a.max_u);
     S0 = lds;
-    S1 = lds + a.max_u;
-    S2 = lds + 2 * a.max_u;
-    S3 = lds + 3 * a.max_u;
-    S4 = lds + 4 * a.max_u;
-    S5 = lds + 5 * a.max_u;
+    S1 = lds + ld;
+    S2 = lds + 2 * ld;
+    S3 = lds + 3 * ld;
+    S4 = lds + 4 * ld;
+    S5 = lds + 5 * ld;
   }
   if (base >= a.n) return;
-  velocity_apply<T, KM>(a, base, in, S0, S1, S2, S3, S4, S5);
+  velocity_apply<T, LDS, KM>(a, base, in, S0, S1, S2, S3, S4, S5);
   // later nodes: each node's streaming operands are requested one node ahead
 #pragma unroll 1
   for (int j = 1; j < CPT; j++) {
@@ -412,7 +420,7 @@ __device__ __forceinline__ void velocity_body(const VelocityArgs<T>& a, int b, u
     if (n >= a.n) break;
     in = next;
     if (j + 1 < CPT) velocity_load<T, LDS, KM>(a, n + bd < a.n ? n + bd : a.n - 1, next);
-    velocity_apply<T, KM>(a, n, in, S0, S1, S2, S3, S4, S5);
+    velocity_apply<T, LDS, KM>(a, n, in, S0, S1, S2, S3, S4, S5);
   }
 }
 
@@ -444,9 +452,9 @@ template <typename T, bool LDS, int KM>
 __device__ __forceinline__ void stress_load(const StressArgs<T>& a, int e, StrIn<T>& in) {
   if constexpr (LDS) {
     if (packed<KM>(a))
-      in.key = load_lkey(a.lkey, e, in.c);
+      in.key = load_lkey(a.lkey, e, in.c, -1);  // a cell has all 8 vertices
     else
-      load_lconn(a.lconn, e, in.c);
+      load_lconn(a.lconn, e, in.c, -1);
   } else {
     load_conn(a.cnode, e, in.c);
   }
@@ -471,86 +479,153 @@ __device__ __forceinline__ void stress_load(const StressArgs<T>& a, int e, StrIn
   in.s[5] = a.S5[e];
 }
 
-// S += dt*(C : grad V + m) at cell e, V gathered from V0..V2 (LDS or HBM).
-template <typename T, int KM>
-__device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrIn<T>& in, const T* V0,
-                                             const T* V1, const T* V2) {
-  // coefficients: the cell's key into the class tables, or the per-cell arrays (special
-  // cells NaN-boxed)
-  T sx, sy, sz;
-  T cc[21];
-  int sp = -1;
-  if (keyed<KM>(a)) {
-    const uint32_t k = resolve_key(a, in.key, sp);
-    const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
-    if (__all(k == k0)) {  // one class for the whole wavefront: scalar loads
-      const auto row = as_const(a.cls + (size_t)(k0 >> kMatShift) * kMatRow);
-#pragma unroll
-      for (int j = 0; j < 21; j++) cc[j] = row[j];
-      axis_spacings(as_const(a.axis), k0, sx, sy, sz);
-    } else {
-      T row[kMatRow];
-      load_row(a.cls + (size_t)(k >> kMatShift) * kMatRow, row);
-#pragma unroll
-      for (int j = 0; j < 21; j++) cc[j] = row[j];
-      axis_spacings(a.axis, k, sx, sy, sz);
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < 21; j++) cc[j] = in.cc[j];
-    sx = in.sx;
-    sy = in.sy;
-    sz = in.sz;
-    if (is_boxed(sx)) {
-      sp = unbox(sx);
-      sx = a.sp_dcx[sp];
-    }
-  }
-  // delV[i][k] = D_i V_k (i: derivative axis, k: velocity component)
-  T f[8];
-  load8_nodes(V0, in.c, f);
-  const T d00 = dscale(diffsum<0>(f), sx);
-  const T d10 = dscale(diffsum<1>(f), sy);
-  const T d20 = dscale(diffsum<2>(f), sz);
-  load8_nodes(V1, in.c, f);
-  const T d01 = dscale(diffsum<0>(f), sx);
-  const T d11 = dscale(diffsum<1>(f), sy);
-  const T d21 = dscale(diffsum<2>(f), sz);
-  load8_nodes(V2, in.c, f);
-  const T d02 = dscale(diffsum<0>(f), sx);
-  const T d12 = dscale(diffsum<1>(f), sy);
-  const T d22 = dscale(diffsum<2>(f), sz);
-  const T e12 = d12 + d21, e02 = d02 + d20, e01 = d01 + d10;
-
-  double m[6] = {0, 0, 0, 0, 0, 0};
-  if (sp >= 0) {
-    const int* off = a.sp_off + 7 * (size_t)sp;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-      double acc = 0.0;
-      for (int t = off[k]; t < off[k + 1]; t++) acc += a.amp[a.term[t]];
-      m[k] = acc;
-    }
-  }
-  T* s = in.s;
+// S += dt*(C : grad V + m) for one cell from its strain-rate terms; cc(j) is the j-th of
+// the 21 stored C (a class-table row or the cell's own array).
+template <typename T, typename R>
+__device__ __forceinline__ void voigt_update(T dt, R cc, T d00, T d11, T d22, T e12, T e02, T e01,
+                                             const double (&m)[6], int sp, T (&s)[6]) {
 #pragma unroll
   for (int ks = 0; ks < 6; ks++) {
     T acc = cc[tri(ks, 0)] * d00 + cc[tri(ks, 1)] * d11 + cc[tri(ks, 2)] * d22 +
             cc[tri(ks, 3)] * e12 + cc[tri(ks, 4)] * e02 + cc[tri(ks, 5)] * e01;
     if constexpr (sizeof(T) == 8) {
       if (sp >= 0) acc = acc + (T)m[ks];
-      s[ks] = s[ks] + a.dt * acc;  // C is plain C_IJ: S += dt*(C:gradV + m)
+      s[ks] = s[ks] + dt * acc;  // C is plain C_IJ: S += dt*(C:gradV + m)
     } else {
-      if (sp >= 0) acc = acc + (T)((double)a.dt * m[ks]);  // C holds dt*C_IJ
+      if (sp >= 0) acc = acc + (T)((double)dt * m[ks]);  // C holds dt*C_IJ
       s[ks] = s[ks] + acc;
     }
   }
-  a.S0[e] = s[0];
-  a.S1[e] = s[1];
-  a.S2[e] = s[2];
-  a.S3[e] = s[3];
-  a.S4[e] = s[4];
-  a.S5[e] = s[5];
+}
+
+// Strain-rate terms of one cell from its 8 vertices: the diagonal D_i V_i and the shear
+// sums D_j V_k + D_k V_j, in the reference's order of operations.
+template <typename T>
+__device__ __forceinline__ void strain_terms(const T* V0, const T* V1, const T* V2, const int (&c)[8], T sx,
+                                             T sy, T sz, T (&d)[6]) {
+  // delV[i][k] = D_i V_k (i: derivative axis, k: velocity component)
+  T f[8];
+  load8_nodes(V0, c, f);
+  const T d00 = dscale(diffsum<0>(f), sx);
+  const T d10 = dscale(diffsum<1>(f), sy);
+  const T d20 = dscale(diffsum<2>(f), sz);
+  load8_nodes(V1, c, f);
+  const T d01 = dscale(diffsum<0>(f), sx);
+  const T d11 = dscale(diffsum<1>(f), sy);
+  const T d21 = dscale(diffsum<2>(f), sz);
+  load8_nodes(V2, c, f);
+  const T d02 = dscale(diffsum<0>(f), sx);
+  const T d12 = dscale(diffsum<1>(f), sy);
+  const T d22 = dscale(diffsum<2>(f), sz);
+  d[0] = d00;
+  d[1] = d11;
+  d[2] = d22;
+  d[3] = d12 + d21;
+  d[4] = d02 + d20;
+  d[5] = d01 + d10;
+}
+
+// Summed moment terms of a special cell, per Voigt component.
+template <typename T>
+__device__ __forceinline__ void cell_moments(const StressArgs<T>& a, int sp, double (&m)[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; k++) m[k] = 0.0;
+  if (sp < 0) return;
+  const int* off = a.sp_off + 7 * (size_t)sp;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    double acc = 0.0;
+    for (int t = off[k]; t < off[k + 1]; t++) acc += a.amp[a.term[t]];
+    m[k] = acc;
+  }
+}
+
+// S += dt*(C : grad V + m) for one cell; cc[j] is the j-th of the 21 stored C (a class-table
+// row or the cell's own array).
+template <typename T, typename R>
+__device__ __forceinline__ void voigt_update(T dt, R cc, const T (&d)[6], const double (&m)[6], int sp,
+                                             T (&s)[6]) {
+#pragma unroll
+  for (int ks = 0; ks < 6; ks++) {
+    T acc = cc[tri(ks, 0)] * d[0] + cc[tri(ks, 1)] * d[1] + cc[tri(ks, 2)] * d[2] +
+            cc[tri(ks, 3)] * d[3] + cc[tri(ks, 4)] * d[4] + cc[tri(ks, 5)] * d[5];
+    if constexpr (sizeof(T) == 8) {
+      if (sp >= 0) acc = acc + (T)m[ks];
+      s[ks] = s[ks] + dt * acc;  // C is plain C_IJ: S += dt*(C:gradV + m)
+    } else {
+      if (sp >= 0) acc = acc + (T)((double)dt * m[ks]);  // C holds dt*C_IJ
+      s[ks] = s[ks] + acc;
+    }
+  }
+}
+
+// S += dt*(C : grad V + m) at cell e, V gathered from V0..V2 (LDS or HBM).
+template <typename T, int KM>
+__device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrIn<T>& in, const T* V0,
+                                             const T* V1, const T* V2) {
+  T d[6];
+  double m[6];
+  int sp = -1;
+  if (keyed<KM>(a)) {
+    // class tables: spacings and material row through the scalar cache where the
+    // wavefront shares them
+    const uint32_t k = resolve_key(a, in.key, sp);
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
+    T sx, sy, sz;
+    if (__all(k == k0))
+      axis_spacings(as_const(a.axis), k0, sx, sy, sz);
+    else
+      axis_spacings(a.axis, k, sx, sy, sz);
+    const uint32_t mat = k >> kMatShift;
+    uint32_t m0 = k0 >> kMatShift;
+    if constexpr (sizeof(T) == 8) {
+      // FP64: one pass per distinct material of the wavefront (a single pass in the common
+      // case), each reading its row through the scalar cache, so the 42 registers of
+      // coefficients never occupy vector registers (a wave more per SIMD)
+      strain_terms(V0, V1, V2, in.c, sx, sy, sz, d);
+      cell_moments(a, sp, m);
+      for (;;) {
+        if (mat == m0) {
+          voigt_update(a.dt, as_const(a.cls + (size_t)m0 * kMatRow), d, m, sp, in.s);
+          break;
+        }
+        m0 = __builtin_amdgcn_readfirstlane(mat);
+      }
+    } else {
+      // FP32: the row requested before the gathers, with scalar loads when the wavefront
+      // has one material
+      T cc[21];
+      if (__all(mat == m0)) {
+        const auto row = as_const(a.cls + (size_t)m0 * kMatRow);
+#pragma unroll
+        for (int j = 0; j < 21; j++) cc[j] = row[j];
+      } else {
+        T row[kMatRow];
+        load_row(a.cls + (size_t)mat * kMatRow, row);
+#pragma unroll
+        for (int j = 0; j < 21; j++) cc[j] = row[j];
+      }
+      strain_terms(V0, V1, V2, in.c, sx, sy, sz, d);
+      cell_moments(a, sp, m);
+      voigt_update(a.dt, cc, d, m, sp, in.s);
+    }
+  } else {
+    // per-cell arrays (special cells NaN-boxed in dcx)
+    T sx = in.sx;
+    if (is_boxed(sx)) {
+      sp = unbox(sx);
+      sx = a.sp_dcx[sp];
+    }
+    strain_terms(V0, V1, V2, in.c, sx, in.sy, in.sz, d);
+    cell_moments(a, sp, m);
+    voigt_update(a.dt, in.cc, d, m, sp, in.s);
+  }
+  a.S0[e] = in.s[0];
+  a.S1[e] = in.s[1];
+  a.S2[e] = in.s[2];
+  a.S3[e] = in.s[3];
+  a.S4[e] = in.s[4];
+  a.S5[e] = in.s[5];
 }
 
 // One workgroup's share of the stress update: chunk b of the instance described by a.
@@ -570,7 +645,7 @@ __device__ __forceinline__ void stress_body(const StressArgs<T>& a, int b, unsig
   if constexpr (LDS) {
     T* lds = reinterpret_cast<T*>(smem);
     const T* const src[3] = {a.V0, a.V1, a.V2};
-    stage<T, 3, U>(lds, a.max_u, a.u_off, a.ulist, b, src);
+    stage<T, 3, U>(lds, a.max_u, a.u_off, a.ulist, b, src, -1);
     V0 = lds;
     V1 = lds + a.max_u;
     V2 = lds + 2 * a.max_u;
@@ -747,7 +822,7 @@ hipError_t launch_velocity(const VelocityArgs<T>& a, int block, hipStream_t s) {
   const bool lds = a.lconn != nullptr || a.lkey != nullptr;
   return with_variant(lds, a.max_u, block, a.cpt, a.lkey ? 3 : a.key ? 1 : 0, [&](auto L, auto U, auto C, auto K) {
     return launch1d(velocity_kernel<T, decltype(L)::value, decltype(U)::value, decltype(C)::value, decltype(K)::value>,
-                    a, a.n, block, lds ? (size_t)6 * a.max_u * sizeof(T) : 0, s);
+                    a, a.n, block, lds ? velocity_lds_bytes(a.max_u, sizeof(T)) : 0, s);
   });
 }
 
@@ -804,7 +879,7 @@ hipError_t launch_batch(const BatchDesc<T>* desc, const int32_t* table, int n, i
   if (reinterpret_cast<uintptr_t>(table) % alignof(int2)) return hipErrorInvalidValue;
   const int2* items = reinterpret_cast<const int2*>(table);
   if (block < 64 || block > RSG_MAX_BLOCK || (block % 64) != 0) return hipErrorInvalidValue;
-  const size_t bytes = lds ? (size_t)(velocity ? 6 : 3) * max_u * sizeof(T) : 0;
+  const size_t bytes = !lds ? 0 : velocity ? velocity_lds_bytes(max_u, sizeof(T)) : (size_t)3 * max_u * sizeof(T);
   const bool persist = grid != n;
   return with_variant(lds, max_u, block, cpt, km, [&](auto L, auto U, auto C, auto K) {
     constexpr bool l = decltype(L)::value;
