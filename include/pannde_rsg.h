@@ -117,8 +117,9 @@ int rsg_create(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t p
  * bit-identical).  A zero field takes the default shown. */
 typedef struct {
   int32_t block_velocity;      /* workgroup size of the velocity kernel, 64..512 step 64 (256;
-                                  512 in FP32 for meshes of 3 x 2^20 nodes or more when
-                                  per_thread_velocity is left at 0) */
+                                  512 for meshes of 3 x 2^20 nodes or more when
+                                  per_thread_velocity is left at 0 and the 512-node staged
+                                  sets fit) */
   int32_t block_stress;        /* workgroup size of the stress kernel (256) */
   int32_t per_thread_velocity; /* nodes per thread: 1, 2 or 4 (1) */
   int32_t per_thread_stress;   /* cells per thread: 1, 2 or 4 (2 for meshes of 3 x 2^20 cells
