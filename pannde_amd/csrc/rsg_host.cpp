@@ -1276,9 +1276,19 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
              rsg::velocity_lds_bytes(plan.max_u, h->elt()) <= lds_limit;
     };
     bool fits = h->lds_v && plan_fits();
-    if (h->lds_v && auto_block_v && (!fits || (h->n_nmat >= 0 && plan.max_u > rsg::kMaxPackedSet))) {
-      h->block_v = 256;  // the default size's sets are too large: back to 256-node workgroups
+    auto packs = [&]() { return h->n_nmat >= 0 && plan.max_u <= rsg::kMaxPackedSet; };
+    if (h->lds_v && auto_block_v && (!fits || (h->n_nmat >= 0 && !packs()))) {
+      // the 512-node sets are too large for LDS or for 10-bit links: 256-node workgroups if
+      // their sets are not
+      const bool fits512 = fits, packs512 = fits && packs();
+      LdsPlan plan512 = std::move(plan);
+      h->block_v = 256;
       fits = plan_fits();
+      if (!((fits && !fits512) || (fits && packs() && !packs512))) {
+        h->block_v = 512;
+        plan = std::move(plan512);
+        fits = fits512;
+      }
     }
     if (fits) {
       if ((rc = upload(h.get(), &p, plan.u_off))) return rc;

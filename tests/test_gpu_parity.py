@@ -266,3 +266,28 @@ def test_plate_linearity_and_zero_source_full_size(plate_pair):
     a.force[0].signal_v = 0.0 * a.force[0].signal_v
     V0, S0, _, _ = run_device(a, nsteps=5)
     assert not np.any(V0) and not np.any(S0)
+
+
+def test_velocity_workgroups_fall_back_when_staged_sets_are_large(plate_pair):
+    """On meshes of 3 Mi nodes or more the velocity kernel takes 512-node workgroups, unless
+    their staged cell sets overflow LDS; then 256-node workgroups.  Bricks of one whole
+    layer (x fastest) make 512 consecutive nodes touch about 1500 cells, over the 64 KiB of
+    FP64 LDS.  Results stay bit-identical."""
+    a, _ = plate_pair
+    s = rsg.DeviceSolver.from_case(a)
+    try:
+        t = s.tuning()
+        assert t["block_velocity"] == 512 and t["links_velocity"] == 2  # packed LDS links
+    finally:
+        s.close()
+    ref = run_device(a, nsteps=10)
+    s = rsg.DeviceSolver.from_case(a, tuning=dict(tile=(256, 256, 1)))
+    try:
+        t = s.tuning()
+        assert t["block_velocity"] == 256 and t["links_velocity"] in (1, 2)  # still LDS-staged
+        tm = s.solve(10)
+        V, S = s.get_state()
+        dev = (V, S, tm, s.traces())
+    finally:
+        s.close()
+    assert_same(dev, ref)
