@@ -88,9 +88,10 @@ def create_all(rsg, case_list, precision, threads=8, tuning=None):
         return list(ex.map(lambda c: rsg.DeviceSolver.from_case(c, precision, tuning=tuning), case_list))
 
 
-def batched(rsg, cases, seed, steps_per_block=50):
+def batched(rsg, cases, seed, steps_per_block=50, lanes=None):
     """All instances in one batch for their full step count, FP32 timed; the same batch in
-    FP64 for the per-trace error."""
+    FP64 for the per-trace error.  lanes: the batch's scheduler lanes (None: the library's
+    default)."""
     from pannde_amd import serve
 
     inst = cases.batch_instances(seed=seed)
@@ -102,7 +103,7 @@ def batched(rsg, cases, seed, steps_per_block=50):
         t0 = time.perf_counter()
         solvers = create_all(rsg, inst, prec, tuning=serve.BATCH_TUNING)
         t_create = time.perf_counter() - t0
-        b = rsg.Batch(prec, steps_per_block=steps_per_block)
+        b = rsg.Batch(prec, steps_per_block=steps_per_block, lanes=lanes)
         for s in solvers:
             b.add(s, nsteps)
         b.synchronize()
@@ -199,6 +200,7 @@ def main():
     ap.add_argument("--no-bracket", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-seed", type=int, default=4)
+    ap.add_argument("--batch-lanes", type=int, default=0, help="scheduler lanes of the batch (0: library default)")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-saturated-rate", type=float, default=40.0,
@@ -293,7 +295,7 @@ def main():
 
     if not args.no_batched:
         log("bench: batched FP32 instances (seed %d)" % args.batch_seed)
-        out["batched"] = batched(rsg, cases, args.batch_seed)
+        out["batched"] = batched(rsg, cases, args.batch_seed, lanes=args.batch_lanes or None)
         out["batched"]["ratio_fp32_batched_vs_fp64_plain"] = round(out["batched"]["fp32"]["value"] / value, 3)
 
     if not args.no_served:
