@@ -293,3 +293,23 @@ def test_velocity_workgroups_fall_back_when_staged_sets_are_large(plate_pair):
     finally:
         s.close()
     assert_same(dev, ref)
+
+
+def test_jittered_planes_and_per_cell_materials():
+    """A 10x10x10 block whose node planes are jittered (10 spacing classes per axis, about 11
+    node spacings per axis) and whose C is perturbed per cell (1000 materials): keys that
+    vary in every field within a wavefront.  Results stay bit-identical to the oracle."""
+    rng = np.random.default_rng(7)
+    case = cases.al_block(seed=7, nx=10, ny=10, nz=10, nsteps=20, n_sensors=4)
+    dx = 1.0e-4
+    for coord in (case.x, case.y, case.z):
+        plane = np.rint(coord / dx).astype(np.int64)
+        coord += dx * 0.02 * rng.random(plane.max() + 1)[plane]
+    for f in case.cij:
+        f.data = f.data * (1 + 0.05 * rng.standard_normal(f.data.shape))
+    s = rsg.DeviceSolver.from_case(case)
+    try:
+        assert s.tuning()["cell_classes"] == 1000
+    finally:
+        s.close()
+    assert_same(run_device(case), orc.run_oracle(case))
