@@ -579,17 +579,27 @@ __device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrI
     const uint32_t mat = k >> kMatShift;
     uint32_t m0 = k0 >> kMatShift;
     if constexpr (sizeof(T) == 8) {
-      // FP64: one pass per distinct material of the wavefront (a single pass in the common
-      // case), each reading its row through the scalar cache, so the 42 registers of
-      // coefficients never occupy vector registers (a wave more per SIMD)
+      // FP64: the coefficients stay out of vector registers (a wave more per SIMD).  The
+      // first lane's row is requested into scalar registers before the gathers, so a
+      // wavefront of one material (the common case) has it when its strain rates are done;
+      // otherwise one pass per distinct material, each reading its row through the scalar
+      // cache.
+      const auto row0 = as_const(a.cls + (size_t)m0 * kMatRow);
+      T c0[21];
+#pragma unroll
+      for (int j = 0; j < 21; j++) c0[j] = row0[j];
       strain_terms(V0, V1, V2, in.c, sx, sy, sz, d);
       cell_moments(a, sp, m);
-      for (;;) {
-        if (mat == m0) {
-          voigt_update(a.dt, as_const(a.cls + (size_t)m0 * kMatRow), d, m, sp, in.s);
-          break;
+      if (__all(mat == m0)) {
+        voigt_update(a.dt, c0, d, m, sp, in.s);
+      } else {
+        for (;;) {
+          if (mat == m0) {
+            voigt_update(a.dt, as_const(a.cls + (size_t)m0 * kMatRow), d, m, sp, in.s);
+            break;
+          }
+          m0 = __builtin_amdgcn_readfirstlane(mat);
         }
-        m0 = __builtin_amdgcn_readfirstlane(mat);
       }
     } else {
       // FP32: the row requested before the gathers, with scalar loads when the wavefront
