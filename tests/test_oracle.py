@@ -173,6 +173,18 @@ def test_write_time_schedule():  # TransducerModel.cpp:68-73
         assert steps[k] == n
 
 
+def test_write_time_schedule_exact_cases():
+    """What exact arithmetic fixes: with dt a power of two and write times on multiples of dt
+    (the case makers' arange(NT+1) * interval * dt), time accumulates exactly, so the loop
+    `while (time < write_time) solve()` takes exactly `interval` steps per write and none
+    before the first (write time 0).  A non-strict comparison would take one more step at
+    every write."""
+    for dt, interval, nt in ((0.125, 8, 4), (2.0 ** -20, 37, 6), (1.0, 1, 3)):
+        wt = np.arange(nt + 1) * interval * dt
+        steps = orc.schedule(dt, wt)
+        assert list(steps) == [0] + [interval] * nt
+
+
 # ---------------------------------------------------------------- golden fixtures
 
 @pytest.mark.parametrize("name", golden_names())
