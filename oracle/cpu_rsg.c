@@ -30,6 +30,10 @@
  *                                                  (oracle/_ref, tests/golden fixtures),
  *                                                  plus invariants (rigid translation,
  *                                                  linearity, uniform strain rate)
+ *   driver step schedule (orc_schedule)          -> exact-arithmetic cases (power-of-two dt:
+ *                                                  exactly `interval` steps per write); the
+ *                                                  time it compares is the reference's,
+ *                                                  bitwise, in every golden fixture
  *
  * Build with -O2 -ffp-contract=off (oracle/Makefile) so no multiply-add is fused.
  */
