@@ -97,26 +97,20 @@ def structured_mesh(nx, ny, nz, dx, dy, dz, keep=None, origin=(0.0, 0.0, 0.0)):
         for dj in (0, 1):
             for dk in (0, 1):
                 used[di:di + nx, dj:dj + ny, dk:dk + nz] |= keep
-    # flat indexing throughout (C order: x outer, z inner)
-    sy, sx = nz + 1, (ny + 1) * (nz + 1)
-    flat = np.flatnonzero(used)
-    nid = np.full(used.size, -1, dtype=np.int32)
-    nid[flat] = np.arange(flat.shape[0], dtype=np.int32)
-    ii, rem = np.divmod(flat, sx)
-    jj, kk = np.divmod(rem, sy)
+    # C order throughout (x outer, z inner): np.nonzero lists kept entries in that order
+    ii, jj, kk = np.nonzero(used)
+    nid = np.full(used.shape, -1, dtype=np.int32)
+    nid[used] = np.arange(ii.shape[0], dtype=np.int32)
     x = origin[0] + ii * dx
     y = origin[1] + jj * dy
     z = origin[2] + kk * dz
-    cflat = np.flatnonzero(keep)
-    ci, rem = np.divmod(cflat, ny * nz)
-    cj, ck = np.divmod(rem, nz)
-    base = ci * sx + cj * sy + ck
+    ci, cj, ck = np.nonzero(keep)
     offs = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
     cells = np.empty((ci.shape[0], 8), dtype=np.int32)
     for v, (a, b, c) in enumerate(offs):
-        cells[:, v] = nid[base + (a * sx + b * sy + c)]
-    return (x.astype(np.float64), y.astype(np.float64), z.astype(np.float64), cells,
-            np.stack([ci, cj, ck], axis=1))
+        cells[:, v] = nid[a:a + nx, b:b + ny, c:c + nz][keep]
+    return (np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64), np.asarray(z, dtype=np.float64),
+            cells, np.stack([ci, cj, ck], axis=1))
 
 
 def renumber_nodes(x, y, z, cells, new_of_old):
