@@ -52,7 +52,37 @@ def fixture_cases():
                          crack_depth=0.6e-3, crack_angle=60.0, n_sensors=8)
     c.name = "ti_bracket_crack"
     out.append((c, 2))
+    c = nodal_stiffness_case()
+    out.append((c, 1))
     return out
+
+
+def nodal_stiffness_case():
+    """HostUTModel form with a moment source and every C_IJ a NODE or CONSTANT field: the
+    reference reads point data first (VTKRead.hpp:151-155) and averages a NODE field over
+    the 8 vertices at each cell (HostField.hpp:121-133).  Entries that vanish for an
+    isotropic solid are CONSTANT zeros; the others vary per node (a smooth trend plus a
+    seeded +-5 % perturbation), with small anisotropic couplings in every shear entry."""
+    c = cases.water_column_block(seed=17, nx=9, ny=8, nz=5, nsteps=25)
+    rng = np.random.default_rng(17)
+    nn = c.n_nodes
+    trend = 1.0 + 0.1 * (c.x - c.x.min()) / np.ptp(c.x)
+    lam = cases.AL["lam"] * trend * (1 + rng.uniform(-0.05, 0.05, nn))
+    mu = cases.AL["mu"] * trend * (1 + rng.uniform(-0.05, 0.05, nn))
+    fields = []
+    for i, j in cases.CIJ_PAIRS:
+        if i < 3 and j < 3:
+            v = lam + 2 * mu if i == j else lam.copy()
+            fields.append(cases.Field(cases.FIELD_NODE, v))
+        elif i == j:
+            fields.append(cases.Field(cases.FIELD_NODE, mu.copy()))
+        elif (i, j) in ((3, 4), (3, 5), (4, 5)):
+            fields.append(cases.Field(cases.FIELD_NODE, 0.02 * mu * rng.uniform(-1, 1, nn)))
+        else:
+            fields.append(cases.Field(cases.FIELD_CONSTANT, np.array([0.0])))
+    c.cij = fields
+    c.name = "water_column_nodec"
+    return c
 
 
 def save(case, ranks):
