@@ -36,18 +36,58 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes (each array element touched once per pass) of the reference's data model
-# (per-entity coefficients, 32-byte connectivity), FP64:
-#   stress kernel per cell: C 21x8 + S read+write 6x8x2 + spacings 3x8 + vertices 8x4
-#                           + V gathered 3x8                                  = 344
-#   velocity kernel per node: V read+write 3x8x2 + dt/rho 8 + spacings 3x8 + cell links
-#                           8x4 + S gathered 6x8                              = 160
-#   FP32 form: stress 21x4 + 6x4x2 + 3x4 + 32 + 3x4 = 188; velocity 3x4x2 + 4 + 12 + 32 + 24 = 96
-STRESS_BYTES = {"f64": 344, "f32": 188}
+# The reference's data model (per-entity coefficients, 32-bit connectivity; each element
+# touched once per pass), FP64 bytes per cell (stress kernel) and per node (velocity kernel):
+#   stress: C 21x8 + S read+write 6x8x2 + spacings 3x8 + vertices 8x4 + V gathered 3x8 = 344
+#   velocity: V read+write 3x8x2 + dt/rho 8 + spacings 3x8 + cell links 8x4 + S gathered 6x8 = 160
+#   FP32: stress 188, velocity 96.  This layout moves fewer bytes than that model, so the model is
+#   reported separately and the roofline uses the layout's own algorithmic bytes.
+SURVEY_MODEL_BYTES = {"stress": {"f64": 344, "f32": 188}, "velocity": {"f64": 160, "f32": 96}}
 # Every 8th step's two launches are bracketed by HIP events inside the timed region: events
 # between every launch slowed the stream by 4 % (FP64) to 8 % (FP32).
 TIMING_STRIDE = 8
-VELOCITY_BYTES = {"f64": 160, "f32": 96}
+# the sources the counter traffic in profiles/ was measured with
+KERNEL_SOURCES = ("pannde_amd/csrc/rsg_kernels.hip", "pannde_amd/csrc/rsg_internal.h")
+# The CPU baseline uses at most this many MPI ranks: the host's CPU share for one GPU.
+CPU_SHARE = 16
+
+
+def layout_bytes(tuning, nc, nn, elt):
+    """Algorithmic bytes per launch of this layout (each array element the kernel needs read
+    or written once per pass): (stress, velocity).  Per cell: the link record (16 B packed
+    10-bit links with the key; 16 B 16-bit links + a 4-byte key; 32 B int32 links + key), the
+    per-cell coefficients when there are no class keys (21 C + 3 spacings), S read and written;
+    plus every node's V gathered once.  Per node likewise, with dt/rho + 3 spacings, V read and
+    written, and every cell's S gathered once.  Gather lists and class tables are not counted."""
+    def record(links, keyed):
+        return 16 if links == 2 else (16 if links == 1 else 32) + (4 if keyed else 0)
+
+    keyed_c, keyed_n = tuning["cell_classes"] >= 0, tuning["node_classes"] >= 0
+    stress = nc * (record(tuning["links_stress"], keyed_c) + (0 if keyed_c else 24 * elt) + 12 * elt) + nn * 3 * elt
+    velocity = nn * (record(tuning["links_velocity"], keyed_n) + (0 if keyed_n else 4 * elt) + 6 * elt) + nc * 6 * elt
+    return stress, velocity
+
+
+def kernel_sources_digest():
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def physical_cores():
+    """Physical cores of this host (lscpu), or None."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], stdout=subprocess.PIPE, text=True, timeout=30).stdout
+    except Exception:
+        return None
+    cores = {tuple(l.split(",")) for l in out.splitlines() if l and not l.startswith("#")}
+    return len(cores) or None
 
 
 def log(*a):
@@ -113,13 +153,19 @@ def batched(rsg, cases, seed, steps_per_block=50, lanes=None):
         dt = time.perf_counter() - t0
         assert left == 0
         traces[tag] = [s.traces() for s in solvers]
+        elt = 4 if prec == rsg.RSG_FP32 else 8
+        step_bytes = sum(sum(layout_bytes(s.tuning(), c.n_cells, c.n_nodes, elt)) for s, c in zip(solvers, inst))
         for slot in list(b.members):
             b.remove(slot)
         b.close()
         for s in solvers:
             s.close()
+        gbs = step_bytes * nsteps / dt / 1e9
         out[tag] = {"value": round(cells * nsteps / dt / 1e9, 4), "seconds": round(dt, 3),
-                    "create_seconds": round(t_create, 2)}
+                    "create_seconds": round(t_create, 2),
+                    # both kernels' algorithmic bytes over the wall clock of the whole batch (launch gaps
+                    # and graph set-up included; measured without a profiler)
+                    "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
         log("bench: batched %s: %d instances, %d cells, %d steps in %.3f s -> %.3f Gcell-steps/s (create %.1f s)"
             % (tag, len(inst), cells, nsteps, dt, out[tag]["value"], t_create))
     errs = [float(serve.rel_l2_traces(a, b).max()) for a, b in zip(traces["fp32"], traces["fp64"])]
@@ -167,23 +213,30 @@ def served(rsg, n, rate, seed):
     return out
 
 
-def cpu_baseline(case, ranks, steps):
+def cpu_baseline(case, ranks, steps, warmup=10, repeat=3):
+    """The reference solver (oracle/_ref/ref_rsg, compiled from the reference headers) under
+    mpirun -n ranks: warmup untimed steps, then the median of `repeat` windows of `steps` steps.
+    Falls back to the oracle restatement on one core when the binary is missing."""
     from oracle import oracle_rsg as orc
 
-    sample = "%s mesh (%d cells), %d steps after 1 warm-up step" % (case.name, case.n_cells, steps)
+    sample = ("%s mesh (%d cells): %d warm-up steps, then the median of %d windows of %d steps"
+              % (case.name, case.n_cells, warmup, repeat, steps))
     if orc.have_reference():
         try:
-            _, _, _, _, tim = orc.run_reference(case, nsteps=steps, ranks=ranks, warmup=1, workdir="/tmp")
+            _, _, _, _, tim = orc.run_reference(case, nsteps=steps, ranks=ranks, warmup=warmup, repeat=repeat,
+                                                workdir="/tmp")
             v = case.n_cells * steps / tim["seconds"] / 1e9
             return dict(value=v, unit="Gcell-steps/s", cores=ranks, kind="reference",
-                        sample=sample + ", reference HostRSGTDUT under mpirun -n %d" % ranks)
+                        sample=sample + ", reference HostRSGTDUT under mpirun -n %d" % ranks,
+                        windows_s=[round(w, 4) for w in tim.get("windows", [])])
         except Exception as e:  # fall through to the port
             log("cpu baseline: reference binary failed (%s); timing the oracle port" % e)
     t0 = time.perf_counter()
     orc.run_oracle(case, nsteps=steps)
     dt = time.perf_counter() - t0
     return dict(value=case.n_cells * steps / dt / 1e9, unit="Gcell-steps/s", cores=1, kind="port",
-                sample=sample + ", oracle restatement, one core (includes its set-up)")
+                sample="%s mesh (%d cells), %d steps, oracle restatement, one core (includes its set-up)"
+                % (case.name, case.n_cells, steps))
 
 
 def main():
@@ -192,8 +245,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-ranks", type=int, default=0, help="0: min(16, host cores)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-ranks", type=int, default=0, help="0: the host's physical cores, at most 16")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps per timed window of the plate's CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -225,25 +278,34 @@ def main():
     ms_step = r64["seconds"] * 1e3 / args.steps
     str_avg = r64["str_ms"] / max(1, r64["launches"]) * 1e-3
     vel_avg = r64["vel_ms"] / max(1, r64["launches"]) * 1e-3
-    str_gbs = STRESS_BYTES["f64"] * nc / str_avg / 1e9
-    vel_gbs = VELOCITY_BYTES["f64"] * nn / vel_avg / 1e9
-    step_gbs = (STRESS_BYTES["f64"] * nc + VELOCITY_BYTES["f64"] * nn) / (str_avg + vel_avg) / 1e9
+    tun = r64["tuning"]
+    str_bytes, vel_bytes = layout_bytes(tun, nc, nn, 8)
+    str_gbs = str_bytes / str_avg / 1e9
+    vel_gbs = vel_bytes / vel_avg / 1e9
+    step_gbs = (str_bytes + vel_bytes) / (str_avg + vel_avg) / 1e9
     log("bench: FP64 %.3f Gcell-steps/s, %.4f ms/step; stress %.1f us (%.0f GB/s), velocity %.1f us (%.0f GB/s)"
         % (value, ms_step, str_avg * 1e6, str_gbs, vel_avg * 1e6, vel_gbs))
 
-    # measured HBM bytes per launch of this layout (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated)
-    traffic = None
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated), valid only for
+    # the kernel sources it was measured with
+    traffic, traffic_note = None, "no counter profile for this workload"
     tpath = os.path.join(ROOT, "profiles", "rsg_kernels_traffic_cfrp_plate.json")
     if os.path.exists(tpath) and args.seed == 2:
         with open(tpath) as f:
-            traffic = json.load(f)["hbm_bytes_per_launch"].get("stress_f64")
-    tun = r64["tuning"]
+            prof = json.load(f)
+        if prof.get("kernel_sources_sha256") == kernel_sources_digest():
+            traffic = prof["hbm_bytes_per_launch"].get("stress_f64")
+            traffic_note = "rocprofv3 counters at commit %s, same kernel sources" % prof.get("commit")
+        else:
+            traffic_note = "counter profile is of other kernel sources (commit %s): not used" % prof.get("commit")
     layout = {"coefficients": ("class keys: %d cell materials, %d node materials" % (tun["cell_classes"],
                                                                                    tun["node_classes"])
                                if tun["cell_classes"] >= 0 else "per-entity arrays"),
               "links": {2: "10-bit block-local packed with the key (LDS staging)",
                         1: "16-bit block-local (LDS staging)", 0: "32-bit global"}[tun["links_stress"]],
               "cells_per_thread_stress": tun["per_thread_stress"]}
+    model_str = SURVEY_MODEL_BYTES["stress"]["f64"] * nc
+    model_vel = SURVEY_MODEL_BYTES["velocity"]["f64"] * nn
 
     out = {
         "metric": "Gcell-steps/s FP64 vs FP32 @<=1e-4 trace err; served solves/hr + p50/p99 lat",
@@ -262,16 +324,24 @@ def main():
                    "nodes": nn, "seed": args.seed, "solver": "DeviceSolver FP64 (HostRSGTDUT arithmetic)"},
         "roofline": {"bound": "hbm", "achieved": round(str_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(str_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "stress_kernel<double>", "bytes_per_launch": STRESS_BYTES["f64"] * nc,
+                     "kernel": "stress_kernel<double>",
+                     "bytes_per_launch": str_bytes,
+                     "bytes_model": "this layout's algorithmic bytes: per cell the 16-byte link+key record, "
+                                    "S read and written (96 B); every node's V gathered once (24 B)",
                      "mean_launch_us": round(str_avg * 1e6, 2),
-                     "velocity_kernel": {"achieved": round(vel_gbs, 1), "bytes_per_launch": VELOCITY_BYTES["f64"] * nn,
-                                         "mean_launch_us": round(vel_avg * 1e6, 2)},
+                     "traffic_note": traffic_note,
+                     "traffic_over_algorithmic": round(traffic / str_bytes, 3) if traffic else None,
+                     "traffic_frac": round(traffic / str_avg / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                     "velocity_kernel": {"achieved": round(vel_gbs, 1), "frac": round(vel_gbs / HBM_PEAK_GBS, 4),
+                                         "bytes_per_launch": vel_bytes, "mean_launch_us": round(vel_avg * 1e6, 2)},
                      "full_step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-                     # the layout moves fewer bytes than the reference's data model: measured
-                     # traffic per launch and the bandwidth it achieves
                      "layout": layout,
-                     "traffic_gbs": round(traffic / str_avg / 1e9, 1) if traffic else None,
-                     "traffic_frac": round(traffic / str_avg / 1e9 / HBM_PEAK_GBS, 4) if traffic else None},
+                     # the reference's data model (per-entity coefficients and 32-bit links), which this
+                     # layout does not move: a rate on it exceeds what HBM can carry
+                     "reference_data_model": {"stress_bytes_per_launch": model_str,
+                                              "velocity_bytes_per_launch": model_vel,
+                                              "stress_gbs_equivalent": round(model_str / str_avg / 1e9, 1),
+                                              "note": "not a roofline: this layout moves fewer bytes"}},
         "device_bytes": r64["dev_bytes"],
     }
 
@@ -285,7 +355,7 @@ def main():
         out["fp32"] = {"value": round(v32, 4), "ratio_vs_fp64": round(v32 / value, 3),
                        "max_trace_rel_l2_vs_fp64": float("%.3e" % max(tr_err)),
                        "wavefield_rel_l2_vs_fp64": float("%.3e" % wf_err),
-                       "stress_kernel_gbs": round(STRESS_BYTES["f32"] * nc / s32 / 1e9, 1)}
+                       "stress_kernel_gbs": round(layout_bytes(r32["tuning"], nc, nn, 4)[0] / s32 / 1e9, 1)}
         log("bench: FP32 %.3f Gcell-steps/s (x%.2f), trace err %.2e, wavefield err %.2e"
             % (v32, v32 / value, max(tr_err), wf_err))
 
@@ -308,11 +378,29 @@ def main():
             out["served_saturated"] = served(rsg, args.serve_requests, args.serve_saturated_rate, args.serve_seed)
 
     if not args.no_cpu_baseline:
-        ranks = args.cpu_ranks or max(1, min(16, os.cpu_count() or 1))
-        log("bench: CPU baseline, reference solver under mpirun -n %d, %d steps" % (ranks, args.cpu_steps))
+        phys = physical_cores()
+        ranks = args.cpu_ranks or max(1, min(CPU_SHARE, phys or os.cpu_count() or 1))
+        log("bench: CPU baseline, reference solver under mpirun -n %d (host: %s physical cores), %d warm-up "
+            "steps + median of 3 windows of %d steps" % (ranks, phys, 10, args.cpu_steps))
         cb = cpu_baseline(case, ranks, args.cpu_steps)
+        cb["host_physical_cores"] = phys
+        cb["cores_note"] = ("ranks = the host's physical cores, capped at the %d-core CPU share a one-GPU job has "
+                            "on this host" % CPU_SHARE)
+        cb["gpu_over_cpu"] = round(value / cb["value"], 1)
+        # configuration 1 as the reference configures it: the Al block 40x40x20 under mpirun -n 4
+        c1 = cases.al_block(seed=1)
+        c1_steps = 150
+        log("bench: CPU baseline, Al block %d cells under mpirun -n 4, median of 3 windows of %d steps"
+            % (c1.n_cells, c1_steps))
+        cb1 = cpu_baseline(c1, 4, c1_steps)
+        r1 = run_timed(rsg, c1, rsg.RSG_FP64, 10, c1.nsteps)
+        cb["al_block_40x40x20_n4"] = {"value": round(cb1["value"], 6), "cores": cb1["cores"], "kind": cb1["kind"],
+                                      "sample": cb1["sample"], "windows_s": cb1.get("windows_s"),
+                                      "device_fp64_value": round(c1.n_cells * c1.nsteps / r1["seconds"] / 1e9, 4),
+                                      "device_steps": c1.nsteps}
         out["cpu_baseline"] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in cb.items()}
-        log("bench: CPU baseline %.5f Gcell-steps/s on %d cores (%s)" % (cb["value"], cb["cores"], cb["kind"]))
+        log("bench: CPU baseline %.5f Gcell-steps/s on %d cores (%s); Al block -n 4: %.5f"
+            % (cb["value"], cb["cores"], cb["kind"], cb1["value"]))
     print(json.dumps(out), flush=True)
 
 

@@ -248,16 +248,18 @@ def read_out_file(path):
     return V, S, t, tr
 
 
-def run_reference(case, nsteps=None, ranks=1, warmup=0, V0=None, S0=None, timeout=3600, workdir=None):
+def run_reference(case, nsteps=None, ranks=1, warmup=0, V0=None, S0=None, timeout=3600, workdir=None, repeat=1):
     """Runs oracle/_ref/ref_rsg (the reference HostRSGTDUT / HostUTModel) on ``case``.
-    Returns (V, S, time, traces, timing) with timing = dict(seconds, setup_seconds, ranks)."""
+    Returns (V, S, time, traces, timing) with timing = dict(seconds, setup_seconds, ranks,
+    windows).  repeat > 1 times the nsteps window that many times after the warm-up
+    (seconds = the median window; the outputs are then not those of warmup + nsteps)."""
     if not have_reference():
         raise FileNotFoundError(REF_BIN)
     d = tempfile.mkdtemp(prefix="rsgref_", dir=workdir)
     try:
         cp, op = os.path.join(d, "case.bin"), os.path.join(d, "out.bin")
         write_case_file(cp, case, nsteps, V0, S0)
-        cmd = [REF_BIN, cp, op, "--warmup", str(warmup)]
+        cmd = [REF_BIN, cp, op, "--warmup", str(warmup), "--repeat", str(int(repeat))]
         if ranks > 1:
             cmd = [MPIRUN, "-n", str(ranks)] + cmd
         env = dict(os.environ, OMP_NUM_THREADS="1")
@@ -269,7 +271,7 @@ def run_reference(case, nsteps=None, ranks=1, warmup=0, V0=None, S0=None, timeou
             if line.startswith("REF_TIMING"):
                 for kv in line.split()[1:]:
                     k, v = kv.split("=")
-                    timing[k] = float(v)
+                    timing[k] = [float(x) for x in v.split(",")] if k == "windows" else float(v)
         V, S, t, tr = read_out_file(op)
         return V, S, t, tr, timing
     finally:
