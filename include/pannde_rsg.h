@@ -185,10 +185,18 @@ int rsg_synchronize(rsg_handle h);
 /* Current simulation time (the value solve() last returned) and steps taken. */
 int rsg_get_time(rsg_handle h, double* time, int64_t* steps);
 
-/* Discrete energy of the current state, accumulated in FP64:
- * kinetic = sum_n 0.5*rho_n*|V_n|^2*w_n with w_n = 8*dxn*dyn*dzn, and
- * stress  = sum_c 0.5*|S_c|^2*dxc*dyc*dzc (a norm of the stress field, no compliance). */
-int rsg_energy(rsg_handle h, double* kinetic, double* stress_norm);
+/* Elastic energy of the current state, accumulated in FP64 (no counterpart in the
+ * reference, which computes none; the FP64 accumulation the mixed-precision form keeps):
+ *   kinetic = sum_n 0.5 * rho_n * dxn*dyn*dzn * |V_n|^2
+ *   strain  = sum_c 0.5 * dxc*dyc*dzc * S_c . C_c^-1 . S_c
+ * rho_n: the node density of the velocity update (HostField::atNode, HostField.hpp:142-154);
+ * dxn..dzn, dxc..dzc: RightHex::nodePrep / cellPrep spacings (RightHex.hpp:253-309), so
+ * rho_n*dxn*dyn*dzn is the node's share of the body's mass; C_c: the 6x6 stiffness of
+ * HostRSGTDUT::updateTsAtCell (HostRSGTDUT.hpp:278-293) acting on the strain rates
+ * (d00, d11, d22, d12+d21, d02+d20, d01+d10), inverted per cell in FP64 (the FP32 form
+ * uses its stored dt*C divided by dt).  A cell whose C is not positive definite makes
+ * strain NaN.  Either pointer may be NULL. */
+int rsg_energy(rsg_handle h, double* kinetic, double* strain);
 
 /* Device-kernel timing: with enabled = n > 0 the velocity and stress launches of every n-th
  * step (the first of each n) are bracketed by HIP events on the handle's stream (1: every

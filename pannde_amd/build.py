@@ -40,6 +40,9 @@ def build(verbose=False):
     for part in (1, 2, 3, 4):
         jobs.append([HIPCC] + DEVICE_FLAGS + ["-DRSG_PART=%d" % part, "-c", os.path.join(CSRC, "rsg_kernels.hip"),
                                               "-o", os.path.join(bdir, "rsg_kernels_part%d.o" % part)])
+    # the energy reduction
+    jobs.append([HIPCC] + DEVICE_FLAGS + ["-c", os.path.join(CSRC, "rsg_energy.hip"), "-o",
+                                          os.path.join(bdir, "rsg_energy.o")])
     objs = [cmd[-1] for cmd in jobs]
     procs = []
     for cmd in jobs:
@@ -58,9 +61,6 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs + ["-o", tmp], verbose)
     os.replace(tmp, OUT)
-    # the HBM copy probe (bandwidth ceiling and counter calibration), a separate program
-    _run([HIPCC] + DEVICE_FLAGS[:5] + DEVICE_FLAGS[5:] + [os.path.join(CSRC, "hbm_copy_probe.hip"), "-o",
-                                                          os.path.join(HERE, "hbm_copy_probe")], verbose)
     # the C++ drop-in driver (PanNDE::Model over the C ABI), linked against the library
     inc = os.path.join(ROOT, "include")
     _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + inc, "-I" + os.path.join(inc, "PanNDE"),

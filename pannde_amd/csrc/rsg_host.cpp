@@ -117,10 +117,15 @@ struct Channel {
   Signal sig;
 };
 
+// A device buffer owned by its handle: released on reallocation and with the handle.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
   int alloc(size_t count) {
     release();
     n = count;
@@ -203,8 +208,8 @@ struct rsg_instance {
   uint32_t* s_lkey = nullptr;
   std::vector<uint16_t> s_lconn_h;
   int32_t s_maxu = 0;
-  double* node_w = nullptr;
-  double* cell_w = nullptr;
+  double* node_m = nullptr;    // node masses rho_n*dxn*dyn*dzn (energy)
+  double* cell_vol = nullptr;  // cell volumes dxc*dyc*dzc (energy)
 
   // source / sensor tables
   std::vector<Channel> channels;
@@ -227,7 +232,7 @@ struct rsg_instance {
   DevBuf<double> trace;
   int64_t trace_dev_steps = 0;       // rows currently held on the device
   std::vector<double> trace_host;    // rows flushed to the host
-  DevBuf<double> partial;
+  DevBuf<double> partial;  // energy: [2][kEnergyMaxBlocks] partials, then the two totals
 
   // timing
   int32_t timing = 0;  // 0: off; n > 0: every n-th step's two launches are timed
@@ -991,6 +996,50 @@ int run_block(rsg_instance* h, int kb, const double* amp_dev) {
   return RSG_OK;
 }
 
+// Energy of the handle's state with partials and totals in `work` (2 * kEnergyMaxBlocks + 2
+// doubles of device memory): both kernels on the handle's stream, then the two totals copied
+// out.  Waits for the batch stream first when the handle is in a batch.
+template <typename T>
+void fill_energy_args(rsg_instance* h, rsg::EnergyArgs<T>& a, double* work) {
+  rsg::VelocityArgs<T> va;
+  fill_args<T>(h, va, a.s);
+  a.V0 = (const T*)h->V[0];
+  a.V1 = (const T*)h->V[1];
+  a.V2 = (const T*)h->V[2];
+  a.node_m = h->node_m;
+  a.nn = h->nn;
+  a.cell_vol = h->cell_vol;
+  a.inv_scale = h->precision == RSG_FP64 ? 1.0 : 1.0 / h->dt;
+  a.partial = work;
+}
+
+int energy_on_stream(rsg_instance* h, double* work, double* kinetic, double* strain) {
+  if (h->batch_stream) HIP_TRY(hipStreamSynchronize(h->batch_stream));
+  if (h->tables_dirty) {
+    int rc = build_tables(h);  // the special-cell keys the kernel resolves
+    if (rc) return rc;
+  }
+  const int nb = rsg::kEnergyMaxBlocks;
+  double* out = work + 2 * nb;
+  hipError_t e;
+  if (h->precision == RSG_FP64) {
+    rsg::EnergyArgs<double> a;
+    fill_energy_args<double>(h, a, work);
+    e = rsg::launch_energy_f64(a, nb, out, h->stream);
+  } else {
+    rsg::EnergyArgs<float> a;
+    fill_energy_args<float>(h, a, work);
+    e = rsg::launch_energy_f32(a, nb, out, h->stream);
+  }
+  if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "energy launch: %s", hipGetErrorString(e));
+  double tot[2] = {0.0, 0.0};
+  HIP_TRY(hipMemcpyAsync(tot, out, sizeof(tot), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (kinetic) *kinetic = tot[0];
+  if (strain) *strain = tot[1];
+  return RSG_OK;
+}
+
 int check_mesh(const rsg_mesh_desc* m) {
   if (!m) return fail(RSG_ERR_ARG, "mesh is null");
   if (m->n_nodes <= 0 || m->n_cells <= 0)
@@ -1235,7 +1284,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
           if (bad < 0 || i < bad) bad = i;
           return;
         }
-        w[i] = rho * 8.0 * dnode[0][n] * dnode[1][n] * dnode[2][n];
+        w[i] = rho * dnode[0][n] * dnode[1][n] * dnode[2][n];  // the node's share of the mass
         for (int d = 0; d < 3; d++)
           dn[d][i] = precision == RSG_FP64 ? dnode[d][n] : 0.25 / dnode[d][n];
         for (int k = 0; k < 8; k++) {
@@ -1309,7 +1358,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if (h->n_nmat >= 0 && !packed)
       if ((rc = h->dalloc((void**)&h->nkey, (size_t)nn * 4))) return rc;
     if ((rc = upload(h.get(), &p, w))) return rc;
-    h->node_w = (double*)p;
+    h->node_m = (double*)p;
     return RSG_OK;
   };
   auto cell_part = [&]() -> int {
@@ -1380,7 +1429,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if (h->n_cmat >= 0 && !packed)
       if ((rc = h->dalloc((void**)&h->ckey, (size_t)nc * 4))) return rc;
     if ((rc = upload(h.get(), &p, w))) return rc;
-    h->cell_w = (double*)p;
+    h->cell_vol = (double*)p;
     return RSG_OK;
   };
   {
@@ -1421,7 +1470,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if ((rc = h->dalloc(&h->S[s], (size_t)nc * h->elt()))) return rc;
     HIP_TRY(hipMemsetAsync(h->S[s], 0, (size_t)nc * h->elt(), h->stream));
   }
-  if ((rc = h->partial.alloc(2 * 1024))) return rc;
+  if ((rc = h->partial.alloc(2 * rsg::kEnergyMaxBlocks + 2))) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));  // this handle's work only (others may be running)
   CREATE_PHASE("state");
   *out = h.release();
@@ -1621,48 +1670,10 @@ int rsg_get_time(rsg_handle h, double* time, int64_t* steps) {
   return RSG_OK;
 }
 
-int rsg_energy(rsg_handle h, double* kinetic, double* stress_norm) {
+int rsg_energy(rsg_handle h, double* kinetic, double* strain) {
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
-  if (h->batch_stream) HIP_TRY(hipStreamSynchronize(h->batch_stream));
-  const int nb = 1024;
-  hipError_t e;
-  if (h->precision == RSG_FP64) {
-    rsg::EnergyArgs<double> a;
-    a.V0 = (const double*)h->V[0];
-    a.V1 = (const double*)h->V[1];
-    a.V2 = (const double*)h->V[2];
-    a.node_w = h->node_w;
-    a.nn = h->nn;
-    for (int s = 0; s < 6; s++) a.S[s] = (const double*)h->S[s];
-    a.cell_w = h->cell_w;
-    a.nc = h->nc;
-    a.partial = h->partial.p;
-    e = rsg::launch_energy_f64(a, nb, h->stream);
-  } else {
-    rsg::EnergyArgs<float> a;
-    a.V0 = (const float*)h->V[0];
-    a.V1 = (const float*)h->V[1];
-    a.V2 = (const float*)h->V[2];
-    a.node_w = h->node_w;
-    a.nn = h->nn;
-    for (int s = 0; s < 6; s++) a.S[s] = (const float*)h->S[s];
-    a.cell_w = h->cell_w;
-    a.nc = h->nc;
-    a.partial = h->partial.p;
-    e = rsg::launch_energy_f32(a, nb, h->stream);
-  }
-  if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "energy launch: %s", hipGetErrorString(e));
-  std::vector<double> p(2 * nb);
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  HIP_TRY(hipMemcpy(p.data(), h->partial.p, p.size() * sizeof(double), hipMemcpyDeviceToHost));
-  double k = 0, s = 0;
-  for (int i = 0; i < nb; i++) {
-    k += p[i];
-    s += p[nb + i];
-  }
-  if (kinetic) *kinetic = k;
-  if (stress_norm) *stress_norm = s;
-  return RSG_OK;
+  int rc = energy_on_stream(h, h->partial.p, kinetic, strain);
+  return rc;
 }
 
 int rsg_set_timing(rsg_handle h, int32_t enabled) {
@@ -1809,7 +1820,17 @@ struct rsg_batch_s {
     std::vector<Launch> layout, graph_layout;
     hipGraph_t graph = nullptr;  // the lane's K-step block, replayed while its layout holds
     hipGraphExec_t exec = nullptr;
+    // graphs replaced while their last launch may still run on the stream: each is destroyed
+    // once an event recorded behind that launch has completed
+    struct Retired {
+      hipGraphExec_t exec;
+      hipGraph_t graph;
+      hipEvent_t done;
+    };
+    std::vector<Retired> retired;
 
+    // Destroys the graph now: only when nothing of it is queued (the stream is idle or the
+    // graph was never launched).
     void drop_graph() {
       if (exec) (void)hipGraphExecDestroy(exec);
       if (graph) (void)hipGraphDestroy(graph);
@@ -1817,8 +1838,42 @@ struct rsg_batch_s {
       graph = nullptr;
       graph_layout.clear();
     }
+    // Sets the graph aside until the work queued so far on the stream has finished.
+    int retire_graph() {
+      if (exec || graph) {
+        hipEvent_t e = nullptr;
+        hipError_t err = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        if (err == hipSuccess) err = hipEventRecord(e, stream);
+        if (err != hipSuccess) {
+          if (e) (void)hipEventDestroy(e);
+          return fail(RSG_ERR_DEVICE, "batch graph retirement: %s", hipGetErrorString(err));
+        }
+        retired.push_back({exec, graph, e});
+      }
+      exec = nullptr;
+      graph = nullptr;
+      graph_layout.clear();
+      return RSG_OK;
+    }
+    // Destroys the retired graphs whose launches have finished (all of them with wait).
+    void reap(bool wait) {
+      size_t keep = 0;
+      for (size_t i = 0; i < retired.size(); i++) {
+        Retired& r = retired[i];
+        const bool finished = wait ? hipEventSynchronize(r.done) == hipSuccess : hipEventQuery(r.done) == hipSuccess;
+        if (finished || wait) {
+          if (r.exec) (void)hipGraphExecDestroy(r.exec);
+          if (r.graph) (void)hipGraphDestroy(r.graph);
+          (void)hipEventDestroy(r.done);
+        } else {
+          retired[keep++] = r;
+        }
+      }
+      retired.resize(keep);
+    }
     void release() {
       if (stream) (void)hipStreamSynchronize(stream);
+      reap(true);
       drop_graph();
       if (ctl_dev) (void)hipFree(ctl_dev);
       ctl_dev = nullptr;
@@ -1959,8 +2014,12 @@ int batch_launch_block(rsg_batch_s* b, rsg_batch_s::Lane& ln) {
     if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "batch launch: %s", hipGetErrorString(e));
     return RSG_OK;
   }
+  ln.reap(false);
   if (!ln.exec || ln.graph_layout != ln.layout) {
-    ln.drop_graph();
+    // the previous block's graph may still be running (this block's control upload is queued
+    // behind it): it is destroyed only after that launch has finished
+    int rc = ln.retire_graph();
+    if (rc) return rc;
     HIP_TRY(hipGraphCreate(&ln.graph, 0));
     hipGraphNode_t node = nullptr;
     hipError_t e = emit(ln.graph, &node);

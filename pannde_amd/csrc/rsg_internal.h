@@ -126,18 +126,19 @@ struct StressArgs {
   const uint32_t* lkey;    // [n][4] LDS links + key, as for VelocityArgs
 };
 
-// Energy reduction arguments (FP64 accumulation, one partial per block).
+// Energy arguments (rsg_energy.hip): FP64 accumulation, one partial per workgroup.
+constexpr int kEnergyMaxBlocks = 1024;
 template <typename T>
 struct EnergyArgs {
   const T* V0;
   const T* V1;
   const T* V2;
-  const double* node_w;  // rho_n * 8*dxn*dyn*dzn (FP64)
+  const double* node_m;    // node mass rho_n * dxn*dyn*dzn (FP64)
   int32_t nn;
-  const T* S[6];
-  const double* cell_w;  // dxc*dyc*dzc
-  int32_t nc;
-  double* partial;  // [2][nblocks]
+  StressArgs<T> s;         // the stress kernel's arguments: S, n and the coefficient form
+  const double* cell_vol;  // dxc*dyc*dzc (FP64)
+  double inv_scale;        // 1 (FP64: C stored as C_IJ) or 1/dt (FP32: stored as dt*C_IJ)
+  double* partial;         // [2][nblocks]
 };
 
 // One instance of a batch: its argument blocks plus where this K-step block's
@@ -173,8 +174,10 @@ hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* table, in
 hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* table, int n, int k, int grid, int block,
                             bool lds, int max_u, int cpt, int km, bool velocity, hipStream_t s, hipGraph_t graph,
                             hipGraphNode_t* node);
-hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s);
-hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nblocks, hipStream_t s);
+// Energy: partials into a.partial, then their fixed-order sum into out[0] (kinetic) and
+// out[1] (strain); both device addresses.  nblocks <= kEnergyMaxBlocks.
+hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, double* out, hipStream_t s);
+hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nblocks, double* out, hipStream_t s);
 
 // NaN boxing of a special-table index.
 inline double box_f64(int32_t idx) {

@@ -747,40 +747,6 @@ __global__ __launch_bounds__(RSG_MAX_BLOCK) RSG_STRESS_ATTR void stress_batch_ke
   }
 }
 
-// Energy: one FP64 partial per block for kinetic and stress-norm sums.
-template <typename T>
-__global__ __launch_bounds__(256) void energy_kernel(EnergyArgs<T> a) {
-  __shared__ double sk[256], ss[256];
-  double k = 0, s = 0;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < a.nn; i += gridDim.x * 256) {
-    const double v0 = a.V0[i], v1 = a.V1[i], v2 = a.V2[i];
-    k += 0.5 * a.node_w[i] * (v0 * v0 + v1 * v1 + v2 * v2);
-  }
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < a.nc; i += gridDim.x * 256) {
-    double q = 0;
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-      const double x = a.S[j][i];
-      q += x * x;
-    }
-    s += 0.5 * a.cell_w[i] * q;
-  }
-  sk[threadIdx.x] = k;
-  ss[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      sk[threadIdx.x] += sk[threadIdx.x + w];
-      ss[threadIdx.x] += ss[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    a.partial[blockIdx.x] = sk[0];
-    a.partial[gridDim.x + blockIdx.x] = ss[0];
-  }
-}
-
 template <typename Args, typename K>
 hipError_t launch1d(K kernel, const Args& a, int n, int block, size_t lds, hipStream_t s) {
   if (n <= 0) return hipSuccess;
@@ -861,10 +827,6 @@ hipError_t launch_velocity_f64(const VelocityArgs<double>& a, int block, hipStre
 hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t s) {
   return launch_stress<double>(a, block, s);
 }
-hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(energy_kernel<double>, dim3(nblocks), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
 #endif
 
 #if RSG_PART == 0 || RSG_PART == 2
@@ -873,10 +835,6 @@ hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStrea
 }
 hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t s) {
   return launch_stress<float>(a, block, s);
-}
-hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(energy_kernel<float>, dim3(nblocks), dim3(256), 0, s, a);
-  return hipGetLastError();
 }
 #endif
 
