@@ -12,7 +12,12 @@ arrive as a seeded Poisson process.  The server
   where they share every launch with the solves already resident, and retires each solve
   when its own step count is reached (its fields stay resident until then);
 * re-runs a seeded fraction of the requests in FP64 (a second batch on its own stream)
-  and states each checked result's relative L2 sensor-trace error against it.
+  and states each checked result's relative L2 sensor-trace error against it;
+* isolates failures: a request whose preparation raises (a mesh the spacing rule rejects,
+  device memory exhausted) or whose result is not finite (traces or field energy NaN or
+  infinite) is reported with ``status`` "failed" and its error, and a request cancelled with
+  ``cancel(rid)`` leaves the batch at the next K-step boundary with ``status`` "cancelled";
+  either way its budget is released and the other requests run on unchanged.
 
 Latency is the time from a request's arrival to its result being on the host.  Nothing
 here is in the reference, which runs one solve per process
@@ -27,6 +32,7 @@ import dataclasses
 import json
 import math
 import sys
+import threading
 import time
 from typing import Dict, List, Optional
 
@@ -140,7 +146,8 @@ class Server:
     """Continuous-batching server over one rsg batch (plus one FP64 batch for spot checks)."""
 
     def __init__(self, precision=rsg.RSG_FP32, hbm_budget=96 << 30, max_resident_cells=48_000_000,
-                 steps_per_block=50, prep_threads=6, keep_traces=False, log=None, tuning=BATCH_TUNING):
+                 steps_per_block=50, prep_threads=6, keep_traces=False, log=None, tuning=BATCH_TUNING,
+                 on_block=None):
         self.precision = precision
         self.tuning = tuning
         self.hbm_budget = int(hbm_budget)
@@ -149,6 +156,20 @@ class Server:
         self.prep_threads = int(prep_threads)
         self.keep_traces = keep_traces
         self.log = log or (lambda *a: None)
+        self.on_block = on_block  # called with the active request ids after each enqueued block
+        self._cancel = set()
+        self._cancel_lock = threading.Lock()
+
+    def cancel(self, rid: int):
+        """Asks the running serve() to drop request ``rid`` (thread-safe): a waiting request is
+        never admitted, one being prepared is discarded when ready, an active one leaves the
+        batch at the next K-step boundary.  Its result gets status "cancelled"."""
+        with self._cancel_lock:
+            self._cancel.add(int(rid))
+
+    def _cancelled(self, rid: int) -> bool:
+        with self._cancel_lock:
+            return rid in self._cancel
 
     def _prepare(self, req: Request, precision: int, case: Optional[cases.Case] = None):
         t0 = time.perf_counter()
@@ -181,6 +202,10 @@ class Server:
         def admit():
             while st["nxt"] < len(reqs) and reqs[st["nxt"]].arrival * time_scale <= now():
                 r = reqs[st["nxt"]]
+                if self._cancelled(r.rid):
+                    finish(r, "cancelled")
+                    st["nxt"] += 1
+                    continue
                 nc = 2 * r.cells  # upper bound before the mesh exists (holes, rounding)
                 est = estimate_device_bytes(nc, int(1.1 * nc), self.precision)
                 busy = st["used_bytes"] > 0 or st["resident_cells"] > 0
@@ -192,14 +217,38 @@ class Server:
                 preps[pool.submit(self._prepare, r, self.precision)] = (r, est, now(), False)
                 st["nxt"] += 1
 
+        def finish(r, status, error=None, **extra):
+            """Records a request that ends without a valid result."""
+            t_done = now()
+            results[r.rid] = dict(rid=r.rid, kind=r.kind, cells=0, nsteps=r.nsteps, arrival=r.arrival * time_scale,
+                                  admitted=t_done, ready=t_done, finished=t_done,
+                                  latency=t_done - r.arrival * time_scale, status=status, error=error, **extra)
+
         def collect(timeout):
             if not preps:
                 return
             done, _ = cf.wait(list(preps), timeout=timeout, return_when=cf.FIRST_COMPLETED)
             for f in done:
                 r, est, t_adm, is_check = preps.pop(f)
-                case, s, secs = f.result()
+                try:
+                    case, s, secs = f.result()
+                except Exception as e:  # this request only: release its share and go on
+                    st["used_bytes"] -= est
+                    if is_check:
+                        results[r.rid]["spot_check_error"] = "%s: %s" % (type(e).__name__, e)
+                        results[r.rid].pop("fp32_traces", None)
+                    else:
+                        st["resident_cells"] -= r.cells
+                        finish(r, "failed", "%s: %s" % (type(e).__name__, e))
+                    self.log("serve: request %d failed in preparation: %s" % (r.rid, e))
+                    continue
                 prep_seconds.append(secs)
+                if not is_check and self._cancelled(r.rid):
+                    st["used_bytes"] -= est
+                    st["resident_cells"] -= r.cells
+                    pool.submit(s.close)
+                    finish(r, "cancelled")
+                    continue
                 job = _Job(req=r, case=case, solver=s, bytes=est, admitted=t_adm, ready=now())
                 if is_check:
                     job.slot = check.add(s, r.nsteps)
@@ -208,6 +257,20 @@ class Server:
                     job.slot = main.add(s, r.nsteps)
                     job.started = now()
                     active[job.slot] = job
+
+        def slot_left(batch, slot):
+            done, total = batch.status(slot)
+            return done < total
+
+        def drop_cancelled():
+            """Active requests cancelled since the last block leave the batch now."""
+            for slot in [k for k, j in active.items() if self._cancelled(j.req.rid) and slot_left(main, k)]:
+                job = active.pop(slot)
+                main.remove(slot)  # waits for its lane's queued blocks
+                st["used_bytes"] -= job.bytes
+                st["resident_cells"] -= job.req.cells
+                pool.submit(job.solver.close)
+                finish(job.req, "cancelled")
 
         def enqueue(batch, jobs):
             """One K-step block if any member has steps left; returns the slots that finish in it."""
@@ -222,17 +285,23 @@ class Server:
             job = active.pop(slot)
             main.remove(slot)  # waits only for the block in which it finished
             tr = job.solver.traces()
+            # a diverged solve (non-finite traces or fields) is reported failed, not returned
+            finite = bool(np.isfinite(tr).all()) and all(math.isfinite(e) for e in job.solver.energy())
             t_done = now()
             r = job.req
             st["cell_steps"] += job.case.n_cells * r.nsteps
             st["resident_cells"] -= r.cells
             results[r.rid] = dict(rid=r.rid, kind=r.kind, cells=job.case.n_cells, nsteps=r.nsteps,
                                   arrival=r.arrival * time_scale, admitted=job.admitted, ready=job.ready,
-                                  finished=t_done, latency=t_done - r.arrival * time_scale)
+                                  finished=t_done, latency=t_done - r.arrival * time_scale,
+                                  status="ok" if finite else "failed",
+                                  error=None if finite else "non-finite result (NaN or Inf in traces or fields)")
+            if not finite:
+                self.log("serve: request %d diverged; reported failed" % r.rid)
             if self.keep_traces:
                 results[r.rid]["traces"] = tr
             pool.submit(job.solver.close)
-            if r.spot_check:  # the FP64 re-run shares the byte budget; its solve reuses the case
+            if r.spot_check and finite:  # the FP64 re-run shares the byte budget; its solve reuses the case
                 est = estimate_device_bytes(job.case.n_cells, job.case.n_nodes, rsg.RSG_FP64)
                 st["used_bytes"] += est - job.bytes
                 preps[pool.submit(self._prepare, r, rsg.RSG_FP64, job.case)] = (r, est, t_done, True)
@@ -264,7 +333,11 @@ class Server:
                             len(reqs) - st["nxt"]))
             admit()
             collect(0)
+            drop_cancelled()
+            blocks0 = st["blocks"]
             new_main = enqueue(main, active)
+            if self.on_block and st["blocks"] > blocks0:
+                self.on_block([j.req.rid for j in active.values()])
             new_check = enqueue(check, checking)
             for slot in pending_main:
                 retire_main(slot)
@@ -288,18 +361,21 @@ class Server:
         main.close()
         check.close()
         pool.shutdown()
-        lat = np.array([results[r.rid]["latency"] for r in reqs])
-        served_cell_steps = sum(results[r.rid]["cells"] * results[r.rid]["nsteps"] for r in reqs)
+        ok = [r for r in reqs if results[r.rid].get("status", "ok") == "ok"]
+        lat = np.array([results[r.rid]["latency"] for r in ok]) if ok else np.zeros(1)
+        served_cell_steps = sum(results[r.rid]["cells"] * results[r.rid]["nsteps"] for r in ok)
         report = dict(
-            requests=len(reqs), elapsed_s=elapsed, solves_per_hour=len(reqs) / elapsed * 3600.0,
+            requests=len(reqs), completed=len(ok), elapsed_s=elapsed, solves_per_hour=len(ok) / elapsed * 3600.0,
             latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),
             latency_mean_s=float(lat.mean()),
             gcell_steps_per_s=served_cell_steps / elapsed / 1e9,
             gcell_steps_per_s_with_checks=st["cell_steps"] / elapsed / 1e9,
             blocks=st["blocks"],
             prep_seconds_mean=float(np.mean(prep_seconds)) if prep_seconds else 0.0,
-            spot_checks=sum(1 for r in reqs if r.spot_check),
+            spot_checks=sum(1 for r in ok if r.spot_check),
             spot_check_max_trace_rel_l2=max(spot_errors) if spot_errors else None,
+            failed=[r.rid for r in reqs if results[r.rid].get("status") == "failed"],
+            cancelled=[r.rid for r in reqs if results[r.rid].get("status") == "cancelled"],
             results=[results[r.rid] for r in reqs])
         return report
 

@@ -61,3 +61,56 @@ def test_byte_budget_admits_one_at_a_time():
     for r in reqs:
         got = [x for x in rep["results"] if x["rid"] == r.rid][0]
         assert np.array_equal(got["traces"], alone_traces(r))
+
+
+class _Faulty(serve.Server):
+    """Host-side injections: request `bad_prep` raises in preparation, request `bad_state`
+    starts from a state holding a NaN (written with set_state), request `cancel_ready` is
+    cancelled while it is prepared."""
+
+    def __init__(self, bad_prep, bad_state, cancel_ready, **kw):
+        super().__init__(**kw)
+        self.bad_prep, self.bad_state, self.cancel_ready = bad_prep, bad_state, cancel_ready
+
+    def _prepare(self, req, precision, case=None):
+        if req.rid == self.bad_prep and precision != rsg.RSG_FP64:
+            raise RuntimeError("injected preparation failure")
+        case, s, secs = super()._prepare(req, precision, case)
+        if req.rid == self.bad_state and precision != rsg.RSG_FP64:
+            V = np.zeros((3, s.n_nodes))
+            V[0, s.n_nodes // 2] = np.nan
+            s.set_state(V, None)
+        if req.rid == self.cancel_ready:
+            self.cancel(req.rid)
+        return case, s, secs
+
+
+def test_failures_and_cancellations_leave_the_other_requests_unchanged():
+    reqs = small_stream(n=9, seed=23, spot=0.0)
+    state = {"cancelled_active": None}
+
+    def on_block(active):
+        # cancel the first request seen active with another block still to come
+        if state["cancelled_active"] is None:
+            for rid in active:
+                if rid not in (1, 3, 5, 7):
+                    srv.cancel(rid)
+                    state["cancelled_active"] = rid
+                    return
+
+    srv = _Faulty(bad_prep=1, bad_state=3, cancel_ready=5, steps_per_block=16, prep_threads=3, keep_traces=True,
+                  on_block=on_block)
+    srv.cancel(7)  # cancelled before it arrives: never admitted
+    rep = srv.serve(reqs)
+    res = {r["rid"]: r for r in rep["results"]}
+    assert res[1]["status"] == "failed" and "injected" in res[1]["error"]
+    assert res[3]["status"] == "failed" and "non-finite" in res[3]["error"]
+    assert res[5]["status"] == "cancelled" and res[7]["status"] == "cancelled"
+    ca = state["cancelled_active"]
+    assert ca is not None and res[ca]["status"] == "cancelled"
+    assert sorted(rep["failed"]) == [1, 3] and sorted(rep["cancelled"]) == sorted([5, 7, ca])
+    ok = [r for r in reqs if r.rid not in (1, 3, 5, 7, ca)]
+    assert rep["completed"] == len(ok) >= 3
+    for r in ok:
+        assert res[r.rid]["status"] == "ok"
+        assert np.array_equal(res[r.rid]["traces"], alone_traces(r)), r.rid
