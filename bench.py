@@ -213,6 +213,40 @@ def served(rsg, n, rate, seed):
     return out
 
 
+def dropin(rsg, case, steps, fast_rate):
+    """The C++ drop-in (pannde_amd/rsg_driver: PanNDE::Model over the C ABI) driven the way
+    TransducerModel drives HostRSGTDUT: time = model->solve() once per step
+    (TransducerModel.cpp:68-73).  Two source forms: the library's signal table, and
+    HostNormalXDZ hidden behind an opaque MultiVariate (evaluated on the host every step,
+    one channel per support cell).  fast_rate: the same job's resident solve(N) rate."""
+    import subprocess
+    import tempfile
+
+    from oracle import oracle_rsg as orc  # only its RSGCASE1 writer
+
+    exe = os.path.join(ROOT, "pannde_amd", "rsg_driver")
+    out = {"cells": case.n_cells, "steps": steps}
+    d = tempfile.mkdtemp(prefix="rsgdrop_", dir="/tmp")
+    try:
+        cp, op = os.path.join(d, "case.bin"), os.path.join(d, "out.bin")
+        orc.write_case_file(cp, case, nsteps=steps)
+        for mode, flags in (("table", []), ("opaque", ["--opaque"])):
+            r = subprocess.run([exe, cp, op] + flags, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                               timeout=900)
+            if r.returncode != 0:
+                raise RuntimeError("rsg_driver failed: %s" % r.stdout[-2000:])
+            secs = float([kv for kv in r.stdout.split() if kv.startswith("seconds=")][0].split("=")[1])
+            rate = case.n_cells * steps / secs / 1e9
+            out[mode] = {"value": round(rate, 4), "ms_per_step": round(secs * 1e3 / steps, 4),
+                         "overhead_us_per_step": round((secs / steps - case.n_cells / fast_rate / 1e9) * 1e6, 1)}
+    finally:
+        import shutil
+
+        shutil.rmtree(d, ignore_errors=True)
+    out["resident_solve_n_value"] = round(fast_rate, 4)
+    return out
+
+
 def cpu_baseline(case, ranks, steps, warmup=10, repeat=3):
     """The reference solver (oracle/_ref/ref_rsg, compiled from the reference headers) under
     mpirun -n ranks: warmup untimed steps, then the median of `repeat` windows of `steps` steps.
@@ -252,6 +286,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-bracket", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--dropin-steps", type=int, default=200, help="plate steps of the drop-in driver leg")
     ap.add_argument("--batch-seed", type=int, default=4)
     ap.add_argument("--batch-lanes", type=int, default=0, help="scheduler lanes of the batch (0: library default)")
     ap.add_argument("--serve-requests", type=int, default=120)
@@ -376,6 +412,15 @@ def main():
             log("bench: served capacity, %d requests at %.2f per second (arrivals outpace the server)"
                 % (args.serve_requests, args.serve_saturated_rate))
             out["served_saturated"] = served(rsg, args.serve_requests, args.serve_saturated_rate, args.serve_seed)
+
+    if not args.no_dropin:
+        log("bench: C++ drop-in driver, one solve() per step (plate %d steps, Al block %d steps)"
+            % (args.dropin_steps, 500))
+        out["dropin"] = {"plate": dropin(rsg, case, args.dropin_steps, value)}
+        c1 = cases.al_block(seed=1)
+        r1 = run_timed(rsg, c1, rsg.RSG_FP64, 10, c1.nsteps)
+        out["dropin"]["al_block"] = dropin(rsg, c1, c1.nsteps, c1.n_cells * c1.nsteps / r1["seconds"] / 1e9)
+        log("bench: drop-in %s" % json.dumps(out["dropin"]))
 
     if not args.no_cpu_baseline:
         phys = physical_cores()

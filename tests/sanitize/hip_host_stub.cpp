@@ -1,0 +1,161 @@
+// hip_host_stub.cpp -- host-memory stand-ins for the HIP runtime calls and kernel launchers
+// that rsg_host.cpp uses, so the library's host code (set-up threads, reordering, class
+// keys, LDS plans, source tables, batch scheduling) can run under AddressSanitizer and
+// ThreadSanitizer on a machine without a GPU.  TEST INFRASTRUCTURE ONLY: "device" memory
+// is host memory, streams and events are tokens, every call completes at once, and the
+// kernel launchers only check their arguments (nothing is computed).
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "../../pannde_amd/csrc/rsg_internal.h"
+
+namespace {
+struct Token {
+  int kind;
+};
+std::atomic<long> g_live{0};  // live allocations, for a leak check at exit
+}  // namespace
+
+extern "C" long stub_live_allocations() { return g_live.load(); }
+
+hipError_t hipMalloc(void** p, size_t bytes) {
+  if (!p) return hipErrorInvalidValue;
+  *p = std::calloc(1, bytes ? bytes : 1);
+  if (!*p) return hipErrorOutOfMemory;
+  g_live++;
+  return hipSuccess;
+}
+hipError_t hipFree(void* p) {
+  if (p) {
+    std::free(p);
+    g_live--;
+  }
+  return hipSuccess;
+}
+hipError_t hipHostMalloc(void** p, size_t bytes, unsigned int) { return hipMalloc(p, bytes); }
+hipError_t hipHostFree(void* p) { return hipFree(p); }
+hipError_t hipMemcpy(void* dst, const void* src, size_t n, hipMemcpyKind) {
+  if (n) std::memcpy(dst, src, n);
+  return hipSuccess;
+}
+hipError_t hipMemcpyAsync(void* dst, const void* src, size_t n, hipMemcpyKind k, hipStream_t) {
+  return hipMemcpy(dst, src, n, k);
+}
+hipError_t hipMemsetAsync(void* dst, int v, size_t n, hipStream_t) {
+  std::memset(dst, v, n);
+  return hipSuccess;
+}
+hipError_t hipGetDevice(int* d) {
+  *d = 0;
+  return hipSuccess;
+}
+hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
+  *v = 64 * 1024;  // the only attribute asked for: LDS bytes per workgroup
+  return hipSuccess;
+}
+const char* hipGetErrorString(hipError_t) { return "stub error"; }
+
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned int) {
+  *s = reinterpret_cast<hipStream_t>(new Token{1});
+  g_live++;
+  return hipSuccess;
+}
+hipError_t hipStreamDestroy(hipStream_t s) {
+  delete reinterpret_cast<Token*>(s);
+  g_live--;
+  return hipSuccess;
+}
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+
+hipError_t hipEventCreate(hipEvent_t* e) {
+  *e = reinterpret_cast<hipEvent_t>(new Token{2});
+  g_live++;
+  return hipSuccess;
+}
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+hipError_t hipEventDestroy(hipEvent_t e) {
+  delete reinterpret_cast<Token*>(e);
+  g_live--;
+  return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
+  *ms = 0.001f;
+  return hipSuccess;
+}
+
+hipError_t hipGraphCreate(hipGraph_t* g, unsigned int) {
+  *g = reinterpret_cast<hipGraph_t>(new Token{3});
+  g_live++;
+  return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g) {
+  delete reinterpret_cast<Token*>(g);
+  g_live--;
+  return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* x, hipGraph_t, hipGraphNode_t*, char*, size_t) {
+  *x = reinterpret_cast<hipGraphExec_t>(new Token{4});
+  g_live++;
+  return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t x) {
+  delete reinterpret_cast<Token*>(x);
+  g_live--;
+  return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipSuccess; }
+
+namespace rsg {
+
+// The launchers check what the real ones check and touch every table entry a kernel would
+// read first (so an out-of-range table shows up under the sanitizer); nothing is computed.
+template <typename A>
+static hipError_t check_block(const A& a, int block) {
+  if (block < 64 || block > 512 || block % 64 || a.cpt < 1) return hipErrorInvalidValue;
+  return hipSuccess;
+}
+hipError_t launch_velocity_f64(const VelocityArgs<double>& a, int block, hipStream_t) { return check_block(a, block); }
+hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStream_t) { return check_block(a, block); }
+hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t) { return check_block(a, block); }
+hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t) { return check_block(a, block); }
+
+template <typename T>
+static hipError_t batch(const BatchDesc<T>* d, const int32_t* table, int n, int block, hipGraph_t graph,
+                        hipGraphNode_t* node) {
+  if (block < 64 || block > 512 || block % 64) return hipErrorInvalidValue;
+  volatile long sink = 0;
+  for (int i = 0; i < n; i++) {
+    const int32_t w = table[2 * i];
+    if (w >= 0) sink = sink + d[w & (kMaxBatchSlots - 1)].kmax;
+  }
+  if (graph && node) *node = reinterpret_cast<hipGraphNode_t>(graph);
+  return hipSuccess;
+}
+hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* t, int n, int, int, int block, bool, int, int,
+                            int, bool, hipStream_t, hipGraph_t g, hipGraphNode_t* node) {
+  return batch(d, t, n, block, g, node);
+}
+hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* t, int n, int, int, int block, bool, int, int,
+                            int, bool, hipStream_t, hipGraph_t g, hipGraphNode_t* node) {
+  return batch(d, t, n, block, g, node);
+}
+hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nb, double* out, hipStream_t) {
+  if (nb <= 0 || nb > kEnergyMaxBlocks || !a.partial || !out) return hipErrorInvalidValue;
+  out[0] = out[1] = 0.0;
+  return hipSuccess;
+}
+hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nb, double* out, hipStream_t) {
+  if (nb <= 0 || nb > kEnergyMaxBlocks || !a.partial || !out) return hipErrorInvalidValue;
+  out[0] = out[1] = 0.0;
+  return hipSuccess;
+}
+
+}  // namespace rsg
