@@ -78,9 +78,24 @@ inline std::chrono::steady_clock::time_point& create_clock() {
       return fail(RSG_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));        \
   } while (0)
 
+// HostUnivariate::evalAt / interpolate (HostUnivariate.hpp:109-142) over m strictly
+// ascending keys: the last value from the last key on, (Dv/Dt)*(t-t_prev)+v_prev below it;
+// false below the first key (undefined in the reference).
+bool signal_at(const double* t, const double* v, size_t m, double at, double* out) {
+  if (!(at < t[m - 1])) {
+    *out = v[m - 1];
+    return true;
+  }
+  const size_t i = (size_t)(std::upper_bound(t, t + m, at) - t);
+  if (i == 0) return false;
+  const double Dt = t[i] - t[i - 1];
+  const double Dv = v[i] - v[i - 1];
+  *out = (Dv / Dt) * (at - t[i - 1]) + v[i - 1];
+  return true;
+}
+
 // HostUnivariate (HostUnivariate.hpp:68-142): keys ascend as in a std::map, a repeated
-// key keeps the value emplaced first; evalAt holds the last value from the last key on
-// and interpolates (Dv/Dt)*(t-t_prev)+v_prev below it.
+// key keeps the value emplaced first; evaluation as signal_at.
 struct Signal {
   std::vector<double> t, v;
   void build(int32_t n, const double* ts, const double* vs) {
@@ -95,19 +110,7 @@ struct Signal {
       v.push_back(vs[i]);
     }
   }
-  bool eval(double at, double* out) const {
-    const size_t m = t.size();
-    if (!(at < t[m - 1])) {
-      *out = v[m - 1];
-      return true;
-    }
-    size_t i = (size_t)(std::upper_bound(t.begin(), t.end(), at) - t.begin());
-    if (i == 0) return false;  // below the first key: undefined in the reference
-    const double Dt = t[i] - t[i - 1];
-    const double Dv = v[i] - v[i - 1];
-    *out = (Dv / Dt) * (at - t[i - 1]) + v[i - 1];
-    return true;
-  }
+  bool eval(double at, double* out) const { return signal_at(t.data(), v.data(), t.size(), at, out); }
 };
 
 struct Channel {
@@ -372,17 +375,24 @@ std::vector<int32_t> locality_order(int32_t n, const std::vector<double>& px,
                                     const std::vector<double>& py, const std::vector<double>& pz,
                                     const double h[3], const double lo[3], const int tile[3]) {
   std::vector<uint32_t> q((size_t)n * 3);
-  uint64_t ext[3] = {1, 1, 1};
-  for (int32_t i = 0; i < n; i++) {
-    const double p[3] = {px[i], py[i], pz[i]};
-    for (int d = 0; d < 3; d++) {
-      double s = std::floor((p[d] - lo[d]) / h[d] + 0.5);
-      if (!(s >= 0)) s = 0;
-      if (s > 2097151.0) s = 2097151.0;
-      q[(size_t)i * 3 + d] = (uint32_t)s;
-      ext[d] = std::max<uint64_t>(ext[d], (uint64_t)s + 1);
+  const int nt = chunk_count(n);
+  std::vector<std::array<uint64_t, 3>> ext_of((size_t)nt, {1, 1, 1});  // per chunk
+  parallel_chunks(n, [&](int w, int64_t b, int64_t e) {
+    auto& ext = ext_of[w];
+    for (int64_t i = b; i < e; i++) {
+      const double p[3] = {px[i], py[i], pz[i]};
+      for (int d = 0; d < 3; d++) {
+        double s = std::floor((p[d] - lo[d]) / h[d] + 0.5);
+        if (!(s >= 0)) s = 0;
+        if (s > 2097151.0) s = 2097151.0;
+        q[(size_t)i * 3 + d] = (uint32_t)s;
+        ext[d] = std::max<uint64_t>(ext[d], (uint64_t)s + 1);
+      }
     }
-  }
+  });
+  uint64_t ext[3] = {1, 1, 1};
+  for (auto& e : ext_of)
+    for (int d = 0; d < 3; d++) ext[d] = std::max(ext[d], e[d]);
   // brick (z, y, x), then (z, y, x) inside the brick; both fit 64 bits for extents < 2^21
   const uint64_t t[3] = {std::min<uint64_t>((uint64_t)tile[0], ext[0]),
                          std::min<uint64_t>((uint64_t)tile[1], ext[1]),
@@ -391,36 +401,56 @@ std::vector<int32_t> locality_order(int32_t n, const std::vector<double>& px,
   std::vector<int32_t> order((size_t)n);
   const long double span = (long double)(nb[0] * t[0]) * (long double)(nb[1] * t[1]) * (long double)(nb[2] * t[2]);
   if (span < 9.2e18L) {
-    // one key per point, brick * (brick volume) + inner < span: a stable LSD radix sort on
-    // the bits the largest key uses keeps equal keys in index order
+    // one key per point, brick * (brick volume) + inner < span: a stable LSD radix sort on the
+    // bits the largest key uses keeps equal keys in index order.  Each pass is split over the
+    // chunks: every chunk counts its digits, chunk-major prefix sums give each chunk its
+    // output range per digit, and each chunk scatters its points in order, so the passes
+    // produce what a serial stable sort produces.
     const uint64_t vol = t[0] * t[1] * t[2];
     std::vector<uint64_t> k((size_t)n), k2((size_t)n);
     std::vector<int32_t> o2((size_t)n);
-    uint64_t kmax = 0;
-    for (int32_t i = 0; i < n; i++) {
-      const uint64_t qx = q[(size_t)i * 3], qy = q[(size_t)i * 3 + 1], qz = q[(size_t)i * 3 + 2];
-      const uint64_t brick = ((qz / t[2]) * nb[1] + qy / t[1]) * nb[0] + qx / t[0];
-      const uint64_t inner = ((qz % t[2]) * t[1] + qy % t[1]) * t[0] + qx % t[0];
-      k[i] = brick * vol + inner;
-      kmax = std::max(kmax, k[i]);
-      order[i] = i;
-    }
+    std::vector<uint64_t> kmax_of((size_t)nt, 0);
+    parallel_chunks(n, [&](int w, int64_t b, int64_t e) {
+      uint64_t m = 0;
+      for (int64_t i = b; i < e; i++) {
+        const uint64_t qx = q[(size_t)i * 3], qy = q[(size_t)i * 3 + 1], qz = q[(size_t)i * 3 + 2];
+        const uint64_t brick = ((qz / t[2]) * nb[1] + qy / t[1]) * nb[0] + qx / t[0];
+        const uint64_t inner = ((qz % t[2]) * t[1] + qy % t[1]) * t[0] + qx % t[0];
+        k[i] = brick * vol + inner;
+        m = std::max(m, k[i]);
+        order[i] = (int32_t)i;
+      }
+      kmax_of[w] = m;
+    });
+    const uint64_t kmax = *std::max_element(kmax_of.begin(), kmax_of.end());
     constexpr int kDigit = 11;
-    std::vector<int32_t> count((size_t)1 << kDigit);
+    constexpr size_t kBins = (size_t)1 << kDigit;
+    const int64_t step = ((int64_t)n + nt - 1) / nt;
+    std::vector<int32_t> count(kBins * nt);
     for (int shift = 0; shift < 64 && (kmax >> shift) != 0; shift += kDigit) {
-      std::fill(count.begin(), count.end(), 0);
-      for (int32_t i = 0; i < n; i++) count[(k[i] >> shift) & ((1u << kDigit) - 1)]++;
+      parallel_tasks(nt, [&](int w) {
+        int32_t* c = &count[kBins * w];
+        std::fill(c, c + kBins, 0);
+        const int64_t b = std::min<int64_t>(n, w * step), e = std::min<int64_t>(n, (w + 1) * step);
+        for (int64_t i = b; i < e; i++) c[(k[i] >> shift) & (kBins - 1)]++;
+      });
       int32_t sum = 0;
-      for (auto& c : count) {
-        const int32_t v = c;
-        c = sum;
-        sum += v;
-      }
-      for (int32_t i = 0; i < n; i++) {
-        const int32_t dst = count[(k[i] >> shift) & ((1u << kDigit) - 1)]++;
-        k2[dst] = k[i];
-        o2[dst] = order[i];
-      }
+      for (size_t dgt = 0; dgt < kBins; dgt++)
+        for (int w = 0; w < nt; w++) {
+          int32_t& c = count[kBins * w + dgt];
+          const int32_t v = c;
+          c = sum;
+          sum += v;
+        }
+      parallel_tasks(nt, [&](int w) {
+        int32_t* c = &count[kBins * w];
+        const int64_t b = std::min<int64_t>(n, w * step), e = std::min<int64_t>(n, (w + 1) * step);
+        for (int64_t i = b; i < e; i++) {
+          const int32_t dst = c[(k[i] >> shift) & (kBins - 1)]++;
+          k2[dst] = k[i];
+          o2[dst] = order[i];
+        }
+      });
       k.swap(k2);
       order.swap(o2);
     }
@@ -454,8 +484,10 @@ bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other
   p.ulist.clear();
   p.lconn.assign((size_t)n * 8, 0xFFFF);
   p.max_u = 0;
-  // contiguous ranges of blocks on their own threads, each with its own generation-stamp set;
-  // the per-range lists are joined in block order
+  // contiguous ranges of blocks on their own threads, each with its own tag array: the block
+  // that last saw an entity (high 32 bits) and the entity's index in that block's set (low
+  // bits), so finding and numbering a link touches one word.  The per-range lists are joined
+  // in block order.
   struct Range {
     std::vector<int32_t> ulist;
     int32_t max_u = 0;
@@ -466,17 +498,18 @@ bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other
   parallel_tasks((int)nr, [&](int w) {
     Range& r = ranges[w];
     const int32_t b0 = (int32_t)std::min<int64_t>(nb, w * step), b1 = (int32_t)std::min<int64_t>(nb, (w + 1) * step);
-    std::vector<int32_t> seen((size_t)n_other, -1), local((size_t)n_other);
+    std::vector<uint64_t> tag((size_t)n_other, ~0ull);
     std::vector<int32_t> ids;
     ids.reserve((size_t)block * 8);
     r.ulist.reserve((size_t)(b1 - b0) * block * 2);
     for (int32_t b = b0; b < b1; b++) {
       const int32_t lo = b * block, hi = std::min(n, lo + block);
+      const uint64_t stamp = (uint64_t)(uint32_t)b << 32;
       ids.clear();
       for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++) {
         const int32_t g = conn[i];
-        if (g >= 0 && seen[g] != b) {
-          seen[g] = b;
+        if (g >= 0 && (tag[g] & ~0xFFFFFFFFull) != stamp) {
+          tag[g] = stamp;
           ids.push_back(g);
         }
       }
@@ -485,9 +518,9 @@ bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other
         r.ok = false;
         return;
       }
-      for (size_t k = 0; k < ids.size(); k++) local[ids[k]] = (int32_t)k;
+      for (size_t k = 0; k < ids.size(); k++) tag[ids[k]] = stamp | k;
       for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++)
-        if (conn[i] >= 0) p.lconn[i] = (uint16_t)local[conn[i]];
+        if (conn[i] >= 0) p.lconn[i] = (uint16_t)(tag[conn[i]] & 0xFFFFFFFFull);
       r.ulist.insert(r.ulist.end(), ids.begin(), ids.end());
       p.u_off[b + 1] = (int32_t)ids.size();  // sizes for now, offsets below
       r.max_u = std::max(r.max_u, (int32_t)ids.size());
@@ -1148,7 +1181,8 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if (c != 1 && c != 2 && c != 4) return fail(RSG_ERR_ARG, "entities per thread %d: must be 1, 2 or 4", c);
 
   CREATE_PHASE("start");
-  // links (HexMesh::buildLinks)
+  // links (HexMesh::buildLinks): slot = vertex ^ 6, the later cell winning a slot two cells
+  // claim, as in the reference's loop
   h->cells.assign(mesh->cell_nodes, mesh->cell_nodes + (size_t)nc * 8);
   h->node_cells.assign((size_t)nn * 8, -1);
   for (int32_t k = 0; k < nc; k++)
@@ -1156,15 +1190,32 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
 
   const double* X[3] = {mesh->x, mesh->y, mesh->z};
   CREATE_PHASE("links");
-  // RightHex::cellPrep
+  // RightHex::cellPrep, and each cell's coordinate bounds for nodePrep below
   std::vector<double> dcell[3];
   for (int d = 0; d < 3; d++) dcell[d].resize(nc);
-  for (int32_t c = 0; c < nc; c++) {
-    const int32_t a = h->cells[(size_t)c * 8], b = h->cells[(size_t)c * 8 + 6];
-    for (int d = 0; d < 3; d++) dcell[d][c] = X[d][b] - X[d][a];
-  }
+  std::vector<double> cbox((size_t)nc * 6);  // per cell: min x, y, z, max x, y, z
+  parallel_ranges(nc, [&](int64_t c0, int64_t c1) {
+    for (int32_t c = (int32_t)c0; c < (int32_t)c1; c++) {
+      const int32_t* v = &h->cells[(size_t)c * 8];
+      for (int d = 0; d < 3; d++) {
+        dcell[d][c] = X[d][v[6]] - X[d][v[0]];
+        double lo = INFINITY, hi = -INFINITY;
+        for (int k = 0; k < 8; k++) {
+          const double p = X[d][v[k]];
+          lo = (p < lo) ? p : lo;
+          hi = (hi < p) ? p : hi;
+        }
+        cbox[(size_t)c * 6 + d] = lo;
+        cbox[(size_t)c * 6 + 3 + d] = hi;
+      }
+    }
+  });
   CREATE_PHASE("cellprep");
-  // RightHex::nodePrep (in parallel; the first node whose spacing fails is reported)
+  // RightHex::nodePrep: the bounds over the node's cells' vertices, scanned from the node's own
+  // coordinate in slot order.  A strict comparison keeps the first value of the extreme in scan
+  // order, and each cell's bounds are the first extreme of its vertices in vertex order, so
+  // taking the cells' bounds in slot order selects the same value as the vertex-by-vertex scan.
+  // The first node whose spacing fails is reported.
   std::vector<double> dnode[3];
   for (int d = 0; d < 3; d++) dnode[d].resize(nn);
   std::mutex bad_mu;
@@ -1177,13 +1228,10 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     for (int kb = 0; kb < 8; kb++) {
       const int32_t c = h->node_cells[(size_t)n * 8 + kb];
       if (c == -1) continue;
-      for (int kbb = 0; kbb < 8; kbb++) {
-        const int32_t m = h->cells[(size_t)c * 8 + kbb];
-        for (int d = 0; d < 3; d++) {
-          const double p = X[d][m];
-          lo[d] = (p < lo[d]) ? p : lo[d];
-          hi[d] = (hi[d] < p) ? p : hi[d];
-        }
+      const double* b = &cbox[(size_t)c * 6];
+      for (int d = 0; d < 3; d++) {
+        lo[d] = (b[d] < lo[d]) ? b[d] : lo[d];
+        hi[d] = (hi[d] < b[3 + d]) ? b[3 + d] : hi[d];
       }
     }
     for (int d = 0; d < 3; d++) {
@@ -1203,6 +1251,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     }
   }
   });
+  std::vector<double>().swap(cbox);
   if (bad_node >= 0) {
     static const char* kAxis[3] = {"DX", "DY", "DZ"};
     return fail(RSG_ERR_MESH, "%s (node %d)", kAxis[bad_axis], (int)bad_node);
@@ -1748,10 +1797,17 @@ int rsg_signal_eval(int32_t n, const double* t, const double* v, int32_t n_eval,
                     double* out) {
   if (n <= 0 || !t || !v || n_eval < 0 || (n_eval && (!at || !out)))
     return fail(RSG_ERR_ARG, "bad argument");
+  // keys already strictly ascending (a table read back from a map): evaluated in place, with
+  // no copy or sort, so per-step callers pay only the search
+  bool ascending = true;
+  for (int32_t i = 1; i < n && ascending; i++) ascending = t[i - 1] < t[i];
   Signal s;
-  s.build(n, t, v);
+  if (!ascending) s.build(n, t, v);
+  const double* kt = ascending ? t : s.t.data();
+  const double* kv = ascending ? v : s.v.data();
+  const size_t m = ascending ? (size_t)n : s.t.size();
   for (int32_t i = 0; i < n_eval; i++)
-    if (!s.eval(at[i], &out[i])) return fail(RSG_ERR_ARG, "time %g precedes the first key", at[i]);
+    if (!signal_at(kt, kv, m, at[i], &out[i])) return fail(RSG_ERR_ARG, "time %g precedes the first key", at[i]);
   return RSG_OK;
 }
 

@@ -6,10 +6,17 @@
 // kernel launchers only check their arguments (nothing is computed).
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <atomic>
+#include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
 
 #include "../../pannde_amd/csrc/rsg_internal.h"
 
@@ -18,6 +25,28 @@ struct Token {
   int kind;
 };
 std::atomic<long> g_live{0};  // live allocations, for a leak check at exit
+
+// With STUB_DIGEST=<file>, every device buffer's size and a hash of its contents when it is
+// freed are written to <file> at exit, sorted: two builds of the set-up code produce the same
+// file exactly when they upload the same arrays.
+struct Digest {
+  std::mutex mu;
+  std::unordered_map<void*, size_t> size;
+  std::vector<std::string> lines;
+  const char* path = std::getenv("STUB_DIGEST");
+  ~Digest() {
+    if (!path) return;
+    std::sort(lines.begin(), lines.end());
+    if (FILE* f = std::fopen(path, "w")) {
+      for (auto& l : lines) std::fprintf(f, "%s\n", l.c_str());
+      std::fclose(f);
+    }
+  }
+};
+Digest& digest() {
+  static Digest d;
+  return d;
+}
 }  // namespace
 
 extern "C" long stub_live_allocations() { return g_live.load(); }
@@ -27,10 +56,29 @@ hipError_t hipMalloc(void** p, size_t bytes) {
   *p = std::calloc(1, bytes ? bytes : 1);
   if (!*p) return hipErrorOutOfMemory;
   g_live++;
+  Digest& d = digest();
+  if (d.path) {
+    std::lock_guard<std::mutex> lock(d.mu);
+    d.size[*p] = bytes;
+  }
   return hipSuccess;
 }
 hipError_t hipFree(void* p) {
   if (p) {
+    Digest& d = digest();
+    if (d.path) {
+      std::lock_guard<std::mutex> lock(d.mu);
+      auto it = d.size.find(p);
+      if (it != d.size.end()) {
+        uint64_t h = 1469598103934665603ull;
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < it->second; i++) h = (h ^ b[i]) * 1099511628211ull;
+        char line[64];
+        std::snprintf(line, sizeof(line), "%zu %016llx", it->second, (unsigned long long)h);
+        d.lines.push_back(line);
+        d.size.erase(it);
+      }
+    }
     std::free(p);
     g_live--;
   }
