@@ -198,6 +198,18 @@ int rsg_get_time(rsg_handle h, double* time, int64_t* steps);
  * strain NaN.  Either pointer may be NULL. */
 int rsg_energy(rsg_handle h, double* kinetic, double* strain);
 
+/* rsg_energy without any allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The same kernels and the same result, bit for bit, with the
+ * partial sums and the two totals in a static device array of 2*1024+2 doubles (16 KiB per
+ * GPU, held until the process ends) instead of the handle's buffer.  Its size does not grow
+ * with the mesh (a fixed 1024 partials), so the only limit is the kernels' indexing: node and
+ * cell counts up to 2^31 - 2^18 - 1, larger ones are rejected with RSG_ERR_ARG.  Every call on
+ * one GPU shares that GPU's array (each call overwrites all of it, nothing needs clearing):
+ * calls that overlap on one GPU race.  Fails with RSG_ERR_STATE when the handle's source
+ * tables would have to be rebuilt (which allocates): call rsg_step or rsg_energy once first. */
+int rsg_energy_noalloc(rsg_handle h, double* kinetic, double* strain);
+
 /* Device-kernel timing: with enabled = n > 0 the velocity and stress launches of every n-th
  * step (the first of each n) are bracketed by HIP events on the handle's stream (1: every
  * step; events between launches slow the stream, so a stride keeps the timed run close to

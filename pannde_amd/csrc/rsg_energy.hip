@@ -21,7 +21,14 @@
 // keeps next to its sensor traces.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "rsg_internal.h"
+
+// Work area of rsg_energy_noalloc: the partials of kEnergyMaxBlocks workgroups and the two
+// totals (16 KiB), one copy per GPU for the life of the process.  Host code never uses this
+// name as an address: energy_scratch() resolves each device's copy with hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[2 * rsg::kEnergyMaxBlocks + 2];
 
 namespace rsg {
 namespace {
@@ -160,6 +167,32 @@ hipError_t launch_energy(const EnergyArgs<T>& a, int nblocks, double* out, hipSt
 }
 
 }  // namespace
+
+hipError_t energy_scratch(int device, double** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices || !out) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    // resolved once per device, with that device current
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) {
+      const hipError_t r = hipSetDevice(cur);
+      if (e == hipSuccess) e = r;
+    }
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    addr[device] = static_cast<double*>(p);
+  }
+  *out = addr[device];
+  return hipSuccess;
+}
 
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, double* out, hipStream_t s) {
   return launch_energy<double>(a, nblocks, out, s);

@@ -156,6 +156,7 @@ constexpr int kTraceCapSteps = 4096;
 
 struct rsg_instance {
   int32_t precision = RSG_FP64;
+  int32_t device = 0;  // the GPU the handle's memory and stream live on
   int32_t nn = 0, nc = 0;
   int32_t moment_form = 0;
   double dt = 0, time = 0;
@@ -1310,6 +1311,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   {
     int dev = 0, bytes = 0;
     HIP_TRY(hipGetDevice(&dev));
+    h->device = dev;
     HIP_TRY(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     lds_limit = (size_t)std::min(bytes, 64 * 1024);
   }
@@ -1723,6 +1725,19 @@ int rsg_energy(rsg_handle h, double* kinetic, double* strain) {
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
   int rc = energy_on_stream(h, h->partial.p, kinetic, strain);
   return rc;
+}
+
+int rsg_energy_noalloc(rsg_handle h, double* kinetic, double* strain) {
+  if (!h) return fail(RSG_ERR_ARG, "handle is null");
+  if (h->nn > rsg::kEnergyMaxEntities || h->nc > rsg::kEnergyMaxEntities)
+    return fail(RSG_ERR_ARG, "%d nodes / %d cells: the allocation-free energy takes at most %d of each", h->nn,
+                h->nc, rsg::kEnergyMaxEntities);
+  if (h->tables_dirty)  // rebuilding them would allocate
+    return fail(RSG_ERR_STATE, "source tables are not built: call rsg_step or rsg_energy once first");
+  double* work = nullptr;
+  const hipError_t e = rsg::energy_scratch(h->device, &work);
+  if (e != hipSuccess) return fail(RSG_ERR_DEVICE, "energy work area: %s", hipGetErrorString(e));
+  return energy_on_stream(h, work, kinetic, strain);
 }
 
 int rsg_set_timing(rsg_handle h, int32_t enabled) {

@@ -177,6 +177,11 @@ hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* table, int
 // Energy: partials into a.partial, then their fixed-order sum into out[0] (kinetic) and
 // out[1] (strain); both device addresses.  nblocks <= kEnergyMaxBlocks.
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nblocks, double* out, hipStream_t s);
+// Device address of the static work area of the allocation-free energy on `device` (2 *
+// kEnergyMaxBlocks + 2 doubles), resolved with hipGetSymbolAddress on first use per device.
+hipError_t energy_scratch(int device, double** out);
+// Largest node or cell count the energy kernels' grid-stride indexing takes.
+constexpr int32_t kEnergyMaxEntities = 0x7FFFFFFF - kEnergyMaxBlocks * 256;
 hipError_t launch_energy_f32(const EnergyArgs<float>& a, int nblocks, double* out, hipStream_t s);
 
 // NaN boxing of a special-table index.

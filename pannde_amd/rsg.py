@@ -35,7 +35,7 @@ EXPORTS = ("rsg_create", "rsg_set_channels", "rsg_set_sensors", "rsg_step", "rsg
            "rsg_get_kernel_times", "rsg_device_bytes", "rsg_destroy", "rsg_last_error",
            "rsg_disc_support", "rsg_signal_eval", "rsg_create_ex", "rsg_get_tuning", "rsg_batch_create",
            "rsg_batch_add", "rsg_batch_remove", "rsg_batch_run", "rsg_batch_synchronize", "rsg_batch_slot_status",
-           "rsg_batch_destroy", "rsg_batch_set_option")
+           "rsg_batch_destroy", "rsg_batch_set_option", "rsg_energy_noalloc")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -111,6 +111,7 @@ def lib():
         L.rsg_synchronize.argtypes = [vp]
         L.rsg_get_time.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int64)]
         L.rsg_energy.argtypes = [vp, _dp, _dp]
+        L.rsg_energy_noalloc.argtypes = [vp, _dp, _dp]
         L.rsg_set_timing.argtypes = [vp, ctypes.c_int32]
         L.rsg_get_kernel_times.argtypes = [vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]
         L.rsg_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]

@@ -195,6 +195,12 @@ hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* t, int n, 
                             int, bool, hipStream_t, hipGraph_t g, hipGraphNode_t* node) {
   return batch(d, t, n, block, g, node);
 }
+hipError_t energy_scratch(int device, double** out) {
+  static double area[2 * kEnergyMaxBlocks + 2];
+  if (device != 0 || !out) return hipErrorInvalidValue;
+  *out = area;
+  return hipSuccess;
+}
 hipError_t launch_energy_f64(const EnergyArgs<double>& a, int nb, double* out, hipStream_t) {
   if (nb <= 0 || nb > kEnergyMaxBlocks || !a.partial || !out) return hipErrorInvalidValue;
   out[0] = out[1] = 0.0;

@@ -149,6 +149,7 @@ void single_instance(const Box& b) {
       EXPECT(rsg_get_traces(h, 0, 301, tr.data()) == RSG_ERR_ARG);
       double ke = -1, se = -1;
       EXPECT(rsg_energy(h, &ke, &se) == RSG_OK);
+      EXPECT(rsg_energy_noalloc(h, &ke, &se) == RSG_OK);
       // caller-supplied amplitudes
       std::vector<double> amp(20 * (k % 2 ? 4 : 1), 0.5);
       EXPECT(rsg_step(h, 20, amp.data(), nullptr) == RSG_OK);
