@@ -59,7 +59,8 @@ def build(verbose=False):
     if failed:
         raise RuntimeError("\n".join(failed))
     tmp = OUT + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs + ["-o", tmp], verbose)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs +
+         ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp], verbose)
     os.replace(tmp, OUT)
     # the C++ drop-in driver (PanNDE::Model over the C ABI), linked against the library
     inc = os.path.join(ROOT, "include")

@@ -15,6 +15,7 @@
 // reference's summation order (evalXds: connected-cell slot, then transducer,
 // HostRSGTDUT.hpp:387-402; evalMoments: vertex, then excitation, HostUTModel.hpp:296-311).
 #include <hip/hip_runtime_api.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <array>
@@ -70,6 +71,15 @@ inline std::chrono::steady_clock::time_point& create_clock() {
   do {                     \
   } while (0)
 #endif
+
+// A roctx range over an entry point's host work (rocprofv3 --marker-trace shows them next to
+// the kernels): set-up, stepping, batch blocks, state and energy transfers.
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
 
 #define HIP_TRY(expr)                                                                    \
   do {                                                                                   \
@@ -1048,6 +1058,7 @@ void fill_energy_args(rsg_instance* h, rsg::EnergyArgs<T>& a, double* work) {
 }
 
 int energy_on_stream(rsg_instance* h, double* work, double* kinetic, double* strain) {
+  Range range_("rsg_energy");
   if (h->batch_stream) HIP_TRY(hipStreamSynchronize(h->batch_stream));
   if (h->tables_dirty) {
     int rc = build_tables(h);  // the special-cell keys the kernel resolves
@@ -1126,6 +1137,7 @@ int rsg_get_tuning(rsg_handle h, rsg_tuning* t) {
 
 int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_t precision,
                   const rsg_tuning* tuning, rsg_handle* out) {
+  Range range_("rsg_create");
   if (!out) return fail(RSG_ERR_ARG, "out is null");
   *out = nullptr;
   int rc = check_mesh(mesh);
@@ -1529,6 +1541,7 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
 }
 
 int rsg_set_channels(rsg_handle h, int32_t n, const rsg_channel_desc* ch) {
+  Range range_("rsg_set_channels");
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
   if (h->in_batch) return fail(RSG_ERR_STATE, "the handle is in a batch; remove it first");
   if (n < 0 || (n > 0 && !ch)) return fail(RSG_ERR_ARG, "bad channel list");
@@ -1588,6 +1601,7 @@ int rsg_set_sensors(rsg_handle h, int32_t n, const int32_t* nodes) {
 }
 
 int rsg_step(rsg_handle h, int32_t nsteps, const double* amplitudes, double* time_out) {
+  Range range_("rsg_step");
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
   if (h->in_batch) return fail(RSG_ERR_STATE, "the handle is stepped by a batch; remove it first");
   if (nsteps < 0) return fail(RSG_ERR_ARG, "nsteps < 0");
@@ -1640,6 +1654,7 @@ int rsg_step(rsg_handle h, int32_t nsteps, const double* amplitudes, double* tim
 }
 
 int rsg_get_state(rsg_handle h, double* V, double* S) {
+  Range range_("rsg_get_state");
   if (!h) return fail(RSG_ERR_ARG, "handle is null");
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->batch_stream) HIP_TRY(hipStreamSynchronize(h->batch_stream));
@@ -2377,6 +2392,7 @@ int rsg_batch_remove(rsg_batch b, int32_t slot) {
 }
 
 int rsg_batch_run(rsg_batch b, int32_t nblocks, int32_t* unfinished) {
+  Range range_("rsg_batch_run");
   if (!b) return fail(RSG_ERR_ARG, "batch is null");
   int32_t left = 0;
   for (auto& sl : b->slots)
