@@ -48,6 +48,8 @@ SURVEY_MODEL_BYTES = {"stress": {"f64": 344, "f32": 188}, "velocity": {"f64": 16
 TIMING_STRIDE = 8
 # the sources the counter traffic in profiles/ was measured with
 KERNEL_SOURCES = ("pannde_amd/csrc/rsg_kernels.hip", "pannde_amd/csrc/rsg_internal.h")
+# Untimed first steps of the drop-in driver leg (code-object loading on first launch).
+DROPIN_UNTIMED = 10
 # The CPU baseline uses at most this many MPI ranks: the host's CPU share for one GPU.
 CPU_SHARE = 16
 
@@ -134,14 +136,24 @@ def batched(rsg, cases, seed, steps_per_block=50, lanes=None):
     default)."""
     from pannde_amd import serve
 
+    from pannde_amd import autotune
+
     inst = cases.batch_instances(seed=seed)
     nsteps = inst[0].nsteps
     cells = sum(c.n_cells for c in inst)
     out = {"instances": len(inst), "cells": cells, "steps": nsteps, "steps_per_block": steps_per_block}
+    # the autotuner's batched choice on 8 instances spread over the size range, for every instance
+    t0 = time.perf_counter()
+    by_size = sorted(inst, key=lambda c: c.n_cells)
+    group = [by_size[(2 * k + 1) * len(by_size) // 16] for k in range(8)]
+    tuning, _ = autotune.tune_batched(group, rsg.RSG_FP32)
+    out["tuning"] = tuning
+    out["tuning_seconds"] = round(time.perf_counter() - t0, 2)
+    log("bench: batched tuning on 8 instances (%d cells): %s" % (sum(c.n_cells for c in group), tuning))
     traces = {}
     for prec, tag in ((rsg.RSG_FP32, "fp32"), (rsg.RSG_FP64, "fp64")):
         t0 = time.perf_counter()
-        solvers = create_all(rsg, inst, prec, tuning=serve.BATCH_TUNING)
+        solvers = create_all(rsg, inst, prec, tuning=tuning)
         t_create = time.perf_counter() - t0
         b = rsg.Batch(prec, steps_per_block=steps_per_block, lanes=lanes)
         for s in solvers:
@@ -199,11 +211,11 @@ def served(rsg, n, rate, seed):
     from pannde_amd import serve
 
     reqs = serve.request_stream(seed=seed, n=n, rate=rate)
-    srv = serve.Server(prep_threads=10, log=log)
+    srv = serve.Server(prep_threads=10, log=log, tuning="auto")
     rep = srv.serve(reqs)
-    keys = ("requests", "elapsed_s", "solves_per_hour", "latency_p50_s", "latency_p99_s", "latency_mean_s",
-            "gcell_steps_per_s", "gcell_steps_per_s_with_checks", "blocks", "prep_seconds_mean",
-            "spot_checks", "spot_check_max_trace_rel_l2")
+    keys = ("requests", "completed", "elapsed_s", "solves_per_hour", "latency_p50_s", "latency_p99_s",
+            "latency_mean_s", "gcell_steps_per_s", "gcell_steps_per_s_with_checks", "blocks", "prep_seconds_mean",
+            "spot_checks", "spot_check_max_trace_rel_l2", "tuning", "tuning_seconds", "failed")
     out = {k: (float("%.4g" % rep[k]) if isinstance(rep[k], float) else rep[k]) for k in keys}
     out.update(arrival_rate_per_s=rate, seed=seed, hbm_budget_gib=srv.hbm_budget / (1 << 30),
                max_resident_cells=srv.max_resident_cells, steps_per_block=srv.K)
@@ -225,20 +237,22 @@ def dropin(rsg, case, steps, fast_rate):
     from oracle import oracle_rsg as orc  # only its RSGCASE1 writer
 
     exe = os.path.join(ROOT, "pannde_amd", "rsg_driver")
-    out = {"cells": case.n_cells, "steps": steps}
+    out = {"cells": case.n_cells, "steps": steps, "untimed_first_steps": DROPIN_UNTIMED}
     d = tempfile.mkdtemp(prefix="rsgdrop_", dir="/tmp")
     try:
         cp, op = os.path.join(d, "case.bin"), os.path.join(d, "out.bin")
         orc.write_case_file(cp, case, nsteps=steps)
         for mode, flags in (("table", []), ("opaque", ["--opaque"])):
-            r = subprocess.run([exe, cp, op] + flags, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
-                               timeout=900)
+            # the first steps load the kernels' code objects: the timer starts after them
+            r = subprocess.run([exe, cp, op, "--time-from", str(DROPIN_UNTIMED)] + flags, stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True, timeout=900)
             if r.returncode != 0:
                 raise RuntimeError("rsg_driver failed: %s" % r.stdout[-2000:])
-            secs = float([kv for kv in r.stdout.split() if kv.startswith("seconds=")][0].split("=")[1])
-            rate = case.n_cells * steps / secs / 1e9
-            out[mode] = {"value": round(rate, 4), "ms_per_step": round(secs * 1e3 / steps, 4),
-                         "overhead_us_per_step": round((secs / steps - case.n_cells / fast_rate / 1e9) * 1e6, 1)}
+            kv = dict(x.split("=", 1) for x in r.stdout.split() if "=" in x)
+            secs, timed = float(kv["seconds"]), int(kv["timed_steps"])
+            rate = case.n_cells * timed / secs / 1e9
+            out[mode] = {"value": round(rate, 4), "ms_per_step": round(secs * 1e3 / timed, 4),
+                         "overhead_us_per_step": round((secs / timed - case.n_cells / fast_rate / 1e9) * 1e6, 1)}
     finally:
         import shutil
 

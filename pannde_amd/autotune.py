@@ -55,3 +55,52 @@ def tune(case, precision=rsg.RSG_FP32, steps=20, warmup=3, blocks=BLOCKS, per_th
     best = dict(block_velocity=bv["block"], per_thread_velocity=bv["per_thread"], lds_velocity=bv["lds"],
                 block_stress=bs["block"], per_thread_stress=bs["per_thread"], lds_stress=bs["lds"])
     return best, results
+
+
+# Choices that change a batch's launches: workgroup size and entities per thread of each
+# kernel (LDS staging stays on: faster on every mesh measured here).
+BATCH_CANDIDATES = [
+    dict(block_velocity=bv, per_thread_velocity=cv, block_stress=bs, per_thread_stress=cs)
+    for (bv, cv) in ((256, 1), (256, 2), (512, 1)) for (bs, cs) in ((256, 1), (256, 2))
+]
+
+
+def time_batched(case, precision, tuning, steps_per_block=25, blocks=4, repeats=3) -> float:
+    """Seconds per step of ``case`` (or a list of cases, stepped together) in a batch with
+    ``tuning``: the batched kernels and their graph-replayed K-step blocks, as a server runs
+    them (median of ``repeats`` runs of ``blocks`` blocks after one warm-up block)."""
+    group = list(case) if isinstance(case, (list, tuple)) else [case]
+    solvers = [rsg.DeviceSolver.from_case(c, precision, tuning=tuning) for c in group]
+    b = rsg.Batch(precision, steps_per_block=steps_per_block)
+    try:
+        slots = [b.add(s, (1 + blocks * repeats) * steps_per_block) for s in solvers]
+        b.run(1)
+        b.synchronize()
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            b.run(blocks)
+            b.synchronize()
+            times.append(time.perf_counter() - t0)
+        for slot in slots:
+            b.remove(slot)
+        times.sort()
+        return times[len(times) // 2] / (blocks * steps_per_block)
+    finally:
+        b.close()
+        for s in solvers:
+            s.close()
+
+
+def tune_batched(case, precision=rsg.RSG_FP32, candidates=None, log=None, **kw) -> Tuple[Dict, List[Dict]]:
+    """The fastest of ``candidates`` (default BATCH_CANDIDATES) for batched solves of meshes
+    like ``case`` (a case, or a list of cases batched together); returns (best tuning,
+    measured candidates)."""
+    results = []
+    for cand in candidates or BATCH_CANDIDATES:
+        sec = time_batched(case, precision, cand, **kw)
+        results.append(dict(cand, seconds_per_step=sec))
+        if log:
+            log("autotune: batched %s: %.4f ms/step" % (cand, sec * 1e3))
+    best = min(results, key=lambda r: r["seconds_per_step"])
+    return {k: v for k, v in best.items() if k != "seconds_per_step"}, results

@@ -123,11 +123,42 @@ def rel_l2_traces(tr, ref):
     return num[keep] / den[keep]
 
 
-# Per-instance tuning for batched solves: two nodes per velocity thread halve the
-# workgroups of every batched velocity launch, and with them the per-workgroup item and
-# descriptor loads (measured on 64 instances of 32^3..128^3 cells: +3 % in FP32, +2 % in
-# FP64).  Results do not depend on it.
+# Per-instance tuning for batched solves when the server does not tune: two nodes per
+# velocity thread halve the workgroups of every batched velocity launch, and with them the
+# per-workgroup item and descriptor loads (measured on 64 instances of 32^3..128^3 cells:
+# +3 % in FP32, +2 % in FP64).  Results do not depend on the tuning.
 BATCH_TUNING = {"per_thread_velocity": 2}
+
+# Size classes of the autotuned server: a request's class is its kind and whether its target
+# cell count is below SIZE_SPLIT; each class is tuned once, on a representative request of
+# REPRESENTATIVE_CELLS[kind][bucket] cells.
+SIZE_SPLIT = 2_000_000
+REPRESENTATIVE_CELLS = {"bracket": (1_000_000, 3_000_000), "plate": (1_000_000, 3_000_000)}
+_TUNED: Dict = {}  # (kind, bucket, precision) -> tuning, shared by the servers of a process
+
+
+def size_class(req: "Request"):
+    return req.kind, int(req.cells >= SIZE_SPLIT)
+
+
+def tune_classes(classes, precision, log=None):
+    """Autotuned batch tuning for each (kind, bucket) in ``classes`` (autotune.tune_batched
+    on a representative request), cached per process; returns {class: tuning}."""
+    from . import autotune
+
+    out = {}
+    for kind, bucket in sorted(classes):
+        key = (kind, bucket, precision)
+        if key not in _TUNED:
+            rep = Request(rid=-1, arrival=0.0, kind=kind, cells=REPRESENTATIVE_CELLS[kind][bucket], nsteps=100,
+                          crack_length=2e-3, crack_depth=1e-3, crack_angle=30.0, xd=(0.5, 0.5), seed=12345,
+                          spot_check=False)
+            best, _ = autotune.tune_batched(build_case(rep), precision)
+            _TUNED[key] = best
+            if log:
+                log("serve: tuned %s %s cells: %s" % (kind, "<2M" if bucket == 0 else ">=2M", best))
+        out[(kind, bucket)] = _TUNED[key]
+    return out
 
 
 @dataclasses.dataclass
@@ -148,8 +179,12 @@ class Server:
     def __init__(self, precision=rsg.RSG_FP32, hbm_budget=96 << 30, max_resident_cells=48_000_000,
                  steps_per_block=50, prep_threads=6, keep_traces=False, log=None, tuning=BATCH_TUNING,
                  on_block=None):
+        """tuning: the solvers' rsg_tuning dict, or "auto": each size class of the stream is
+        autotuned for batched solves when serve() starts (tune_classes; the time is reported
+        as tuning_seconds and is not part of the served clock)."""
         self.precision = precision
         self.tuning = tuning
+        self.class_tuning = {}
         self.hbm_budget = int(hbm_budget)
         self.max_resident_cells = int(max_resident_cells)
         self.K = int(steps_per_block)
@@ -171,11 +206,16 @@ class Server:
         with self._cancel_lock:
             return rid in self._cancel
 
+    def _tuning_for(self, req: Request):
+        if self.tuning == "auto":
+            return self.class_tuning.get(size_class(req), BATCH_TUNING)
+        return self.tuning
+
     def _prepare(self, req: Request, precision: int, case: Optional[cases.Case] = None):
         t0 = time.perf_counter()
         if case is None:
             case = build_case(req)
-        s = rsg.DeviceSolver.from_case(case, precision, tuning=self.tuning)
+        s = rsg.DeviceSolver.from_case(case, precision, tuning=self._tuning_for(req))
         return case, s, time.perf_counter() - t0
 
     def serve(self, requests: List[Request], time_scale: float = 1.0) -> Dict:
@@ -185,6 +225,10 @@ class Server:
         retires the solves that finished in the previous block (whose results only need that
         block), admits and folds in newcomers, and hands solver tear-down to the pool."""
         reqs = sorted(requests, key=lambda r: r.arrival)
+        t_tune = time.perf_counter()
+        if self.tuning == "auto":
+            self.class_tuning = tune_classes({size_class(r) for r in reqs}, self.precision, log=self.log)
+        t_tune = time.perf_counter() - t_tune
         pool = cf.ThreadPoolExecutor(max_workers=self.prep_threads)
         main = rsg.Batch(self.precision, steps_per_block=self.K)
         check = rsg.Batch(rsg.RSG_FP64, steps_per_block=self.K)
@@ -374,6 +418,9 @@ class Server:
             prep_seconds_mean=float(np.mean(prep_seconds)) if prep_seconds else 0.0,
             spot_checks=sum(1 for r in ok if r.spot_check),
             spot_check_max_trace_rel_l2=max(spot_errors) if spot_errors else None,
+            tuning=({"%s_%s" % (k, "lt2M" if b == 0 else "ge2M"): t for (k, b), t in self.class_tuning.items()}
+                    if self.tuning == "auto" else self.tuning),
+            tuning_seconds=t_tune,
             failed=[r.rid for r in reqs if results[r.rid].get("status") == "failed"],
             cancelled=[r.rid for r in reqs if results[r.rid].get("status") == "cancelled"],
             results=[results[r.rid] for r in reqs])
@@ -394,12 +441,14 @@ def main(argv=None):
     ap.add_argument("--max-resident-cells", type=float, default=48e6)
     ap.add_argument("--steps-per-block", type=int, default=50)
     ap.add_argument("--prep-threads", type=int, default=6)
+    ap.add_argument("--no-autotune", action="store_true", help="fixed batch tuning instead of tuning per size class")
     ap.add_argument("--out", default=None, help="JSON report path")
     a = ap.parse_args(argv)
     reqs = request_stream(seed=a.seed, n=a.requests, rate=a.rate, cells=(a.min_cells, a.max_cells),
                           steps=(a.min_steps, a.max_steps), spot_fraction=a.spot_fraction)
     srv = Server(hbm_budget=int(a.hbm_budget_gib * (1 << 30)), max_resident_cells=int(a.max_resident_cells),
                  steps_per_block=a.steps_per_block, prep_threads=a.prep_threads,
+                 tuning=BATCH_TUNING if a.no_autotune else "auto",
                  log=lambda *m: print(*m, file=sys.stderr, flush=True))
     rep = srv.serve(reqs)
     summary = {k: v for k, v in rep.items() if k != "results"}
