@@ -234,14 +234,14 @@ def dropin(rsg, case, steps, fast_rate):
     import subprocess
     import tempfile
 
-    from oracle import oracle_rsg as orc  # only its RSGCASE1 writer
+    from pannde_amd.case_file import write_case_file
 
     exe = os.path.join(ROOT, "pannde_amd", "rsg_driver")
     out = {"cells": case.n_cells, "steps": steps, "untimed_first_steps": DROPIN_UNTIMED}
     d = tempfile.mkdtemp(prefix="rsgdrop_", dir="/tmp")
     try:
         cp, op = os.path.join(d, "case.bin"), os.path.join(d, "out.bin")
-        orc.write_case_file(cp, case, nsteps=steps)
+        write_case_file(cp, case, nsteps=steps)
         for mode, flags in (("table", []), ("opaque", ["--opaque"])):
             # the first steps load the kernels' code objects: the timer starts after them
             r = subprocess.run([exe, cp, op, "--time-from", str(DROPIN_UNTIMED)] + flags, stdout=subprocess.PIPE,

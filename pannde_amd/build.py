@@ -62,11 +62,12 @@ def build(verbose=False):
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs +
          ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp], verbose)
     os.replace(tmp, OUT)
-    # the C++ drop-in driver (PanNDE::Model over the C ABI), linked against the library
+    # the C++ drop-in driver (PanNDE::Model over the C ABI; VTK XML jobs through vtk_job.hpp,
+    # zlib for compressed arrays), linked against the library
     inc = os.path.join(ROOT, "include")
     _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + inc, "-I" + os.path.join(inc, "PanNDE"),
           os.path.join(CSRC, "rsg_driver.cpp"), "-o", os.path.join(HERE, "rsg_driver"), "-L" + HERE,
-          "-lpannde_rsg", "-Wl,-rpath,$ORIGIN"], verbose)
+          "-lpannde_rsg", "-lz", "-Wl,-rpath,$ORIGIN"], verbose)
     return OUT
 
 

@@ -9,8 +9,8 @@ XML format defines them:
 
 * ``ascii``, inline ``binary`` (base64) and ``appended`` data (raw or base64);
 * ``header_type`` UInt32 or UInt64, ``byte_order`` LittleEndian or BigEndian;
-* ``compressor="vtkZLibDataCompressor"`` (block header: number of blocks, block size, last
-  block size, compressed sizes);
+* ``compressor="vtkZLibDataCompressor"`` (block header: number of blocks, block size, size of a
+  partial last block or 0 when the last block is full, compressed sizes);
 * every numeric DataArray type; cells of type 12 (hexahedron) and 11 (voxel, reordered to the
   hexahedron vertex order).
 
@@ -98,7 +98,9 @@ class _Decoder:
             out.append(zlib.decompress(body[pos:pos + int(c)]))
             pos += int(c)
         blob = b"".join(out)
-        expect = (nb - 1) * int(sizes[1]) + int(sizes[2]) if nb else 0
+        # the third header word is the size of a partial last block, 0 when the last block is full
+        last = int(sizes[2]) or int(sizes[1])
+        expect = (nb - 1) * int(sizes[1]) + last if nb else 0
         if len(blob) != expect:
             raise VtkError("compressed block sizes do not match the header")
         return blob

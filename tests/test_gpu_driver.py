@@ -1,7 +1,11 @@
-"""The TransducerModel driver on the DeviceSolver, from a job .vtu to solution and table
-files (serial names, and the partitioned .pvtu/.pvtt layout): every velocity table and the
-last solution equal the oracle run of the same job, bit for bit, with the step counts of the
-reference's write-time schedule."""
+"""The TransducerModel and WaterColumnModel drivers on the DeviceSolver, from a job .vtu to
+solution and table files (serial names, and the partitioned .pvtu/.pvtt layout): every velocity
+table and solution equal the oracle run of the same job, bit for bit, with the step counts of the
+reference's write-time schedule; the C++ driver (rsg_driver -f) writes what the Python driver
+writes."""
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -70,3 +74,73 @@ def test_water_column_job_matches_oracle(tmp_path):
             assert np.array_equal(sol.cell_data[name], S[j]), (k, name)
         assert sol.field_data["time"][0] == t
     assert np.abs(sol.cell_data["Sxx"]).max() > 0
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "pannde_amd", "rsg_driver")
+
+
+def run_cpp(args):
+    print("rsg_driver", " ".join(args), flush=True)
+    r = subprocess.run([DRIVER] + args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:]
+    return r.stdout
+
+
+@pytest.mark.parametrize("partitioned,ascii_out", [(False, False), (True, True)])
+def test_cpp_driver_job_file_equals_python_driver(tmp_path, partitioned, ascii_out):
+    """rsg_driver -f job.vtu -o out (the C++ TransducerModel) writes the same tables and
+    solutions as the Python driver, value for value, with time = solve() once per step; the
+    opaque-source form (sources evaluated on the host every step) writes the same values."""
+    src = cases.al_block(seed=43, nx=10, ny=9, nz=6, nsteps=40, n_sensors=1)
+    wt = np.array([0.0, 6.2, 17.5, 31.0]) * src.dt
+    job = tmp_path / "job.vtu"
+    vtkio.case_to_job(src, str(job), wt, fmt="binary", compress=True)
+    dirs = {k: tmp_path / k for k in ("py", "cpp", "opaque")}
+    for d in dirs.values():
+        d.mkdir()
+    transducer_model.run(str(job), str(dirs["py"] / "sol"), rsg.RSG_FP64, tables_dir=str(dirs["py"]),
+                         partitioned=partitioned)
+    flags = (["--partitioned"] if partitioned else []) + (["--ascii"] if ascii_out else [])
+    for k in ("cpp", "opaque"):
+        out = run_cpp(["-f", str(job), "-o", str(dirs[k] / "sol"), "--tables-dir", str(dirs[k])] + flags
+                      + (["--opaque"] if k == "opaque" else []))
+        assert "done: %d steps" % vtkio.schedule_steps(src.dt, wt)[-1] in out
+    t_ext, g_ext = (".pvtt", ".pvtu") if partitioned else (".vtt", ".vtu")
+    table = vtkio.read_pvtt if partitioned else vtkio.read_vtt
+    grid = vtkio.read_pvtu if partitioned else vtkio.read_vtu
+    names = ["node_coords" + t_ext] + ["velocities_%d%s" % (k, t_ext) for k in range(len(wt))]
+    for k in ("cpp", "opaque"):
+        for n in names:
+            a, b = table(str(dirs["py"] / n)), table(str(dirs[k] / n))
+            assert sorted(a) == sorted(b) and all(np.array_equal(a[c], b[c]) for c in a), (k, n)
+        for w in range(len(wt)):
+            a, b = grid(str(dirs["py"] / ("sol_%d%s" % (w, g_ext)))), grid(str(dirs[k] / ("sol_%d%s" % (w, g_ext))))
+            assert np.array_equal(a.points, b.points) and np.array_equal(a.cells, b.cells)
+            assert a.field_data["time"][0] == b.field_data["time"][0]
+            for name in rsg.STATE_V:
+                assert np.array_equal(a.point_data[name], b.point_data[name]), (k, w, name)
+            for name in rsg.STATE_S:
+                assert np.array_equal(a.cell_data[name], b.cell_data[name]), (k, w, name)
+    assert np.abs(table(str(dirs["cpp"] / names[-1]))["Vz"]).max() > 0
+
+
+def test_cpp_driver_water_column_job_matches_oracle(tmp_path):
+    """rsg_driver -f job.vtu --water-column (the C++ WaterColumnModel): every written solution
+    equals the oracle's at its step count."""
+    from test_vtkio import two_column_case
+
+    src = two_column_case(nsteps=24)
+    wt = np.array([0.0, 9.5, 23.0]) * src.dt
+    job = tmp_path / "wc.vtu"
+    vtkio.case_to_water_column_job(src, str(job), wt, fmt="appended", compress=True)
+    run_cpp(["-f", str(job), "-o", str(tmp_path / "wc_out"), "--water-column"])
+    case, _ = vtkio.water_column_case(str(job))
+    for k, n in enumerate(vtkio.schedule_steps(case.dt, wt)):
+        sol = vtkio.read_vtu(str(tmp_path / ("wc_out_%d.vtu" % k)))
+        V, S, t, _ = orc.run_oracle(case, nsteps=int(n))
+        for j, name in enumerate(rsg.STATE_V):
+            assert np.array_equal(sol.point_data[name], V[j]), (k, name)
+        for j, name in enumerate(rsg.STATE_S):
+            assert np.array_equal(sol.cell_data[name], S[j]), (k, name)
+        assert sol.field_data["time"][0] == t
+    assert np.abs(sol.cell_data["Szz"]).max() > 0
