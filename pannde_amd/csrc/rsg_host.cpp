@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <exception>
 #include <memory>
 #include <mutex>
@@ -53,18 +54,31 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-// Set-up phase timing on stderr, compiled in only with -DRSG_CREATE_TIMING.
+// Set-up phase timing on stderr (wall time, and the process's CPU time over all threads),
+// compiled in only with -DRSG_CREATE_TIMING.
 #ifdef RSG_CREATE_TIMING
 inline std::chrono::steady_clock::time_point& create_clock() {
   static thread_local std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
   return t;
 }
+inline double process_cpu_ms() {
+  timespec ts;
+  clock_gettime(CLOCK_PROCESS_CPUTIME_ID, &ts);
+  return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+inline double& create_cpu() {
+  static thread_local double c = process_cpu_ms();
+  return c;
+}
 #define CREATE_PHASE(name)                                                                        \
   do {                                                                                           \
     const auto t_now_ = std::chrono::steady_clock::now();                                        \
-    fprintf(stderr, "rsg_create %-12s %8.1f ms\n", name,                                         \
-            std::chrono::duration<double, std::milli>(t_now_ - create_clock()).count());         \
+    const double c_now_ = process_cpu_ms();                                                      \
+    fprintf(stderr, "rsg_create %-12s %8.1f ms wall %8.1f ms cpu\n", name,                       \
+            std::chrono::duration<double, std::milli>(t_now_ - create_clock()).count(),          \
+            c_now_ - create_cpu());                                                              \
     create_clock() = t_now_;                                                                     \
+    create_cpu() = c_now_;                                                                       \
   } while (0)
 #else
 #define CREATE_PHASE(name) \
@@ -313,7 +327,7 @@ int check_field(const rsg_field_desc& f, const char* name) {
 }
 
 // HostField::atNode (HostField.hpp:142-154).
-double field_at_node(const rsg_field_desc& f, const int32_t* ncells, int32_t n) {
+__attribute__((always_inline)) inline double field_at_node(const rsg_field_desc& f, const int32_t* ncells, int32_t n) {
   if (f.type == RSG_FIELD_NODE) return f.data[n];
   double v = 0.0;
   for (int k = 0; k < 8; k++) {
@@ -324,7 +338,7 @@ double field_at_node(const rsg_field_desc& f, const int32_t* ncells, int32_t n) 
 }
 
 // HostField::atCell (HostField.hpp:121-133).
-double field_at_cell(const rsg_field_desc& f, const int32_t* cells, int32_t c) {
+__attribute__((always_inline)) inline double field_at_cell(const rsg_field_desc& f, const int32_t* cells, int32_t c) {
   if (f.type == RSG_FIELD_NODE) {
     double v = 0.0;
     for (int k = 0; k < 8; k++) v += f.data[cells[(size_t)c * 8 + k]];
@@ -378,55 +392,135 @@ void parallel_ranges(int64_t n, F&& fn) {
   parallel_chunks(n, [&fn](int, int64_t b, int64_t e) { fn(b, e); });
 }
 
-// Locality order: the point quantised to the finest spacing, (qx, qy, qz), sorted by
-// bricks of tile[0] x tile[1] x tile[2] (brick z, brick y, brick x, then z, y, x inside
-// the brick).  tile = {huge, 1, 1} is plain (z, y, x) lexicographic order: x-runs are
-// contiguous, so each wavefront's gathers touch a few short runs.
-std::vector<int32_t> locality_order(int32_t n, const std::vector<double>& px,
-                                    const std::vector<double>& py, const std::vector<double>& pz,
+// A coordinate quantised to the finest spacing: floor((p - lo) / h + 0.5) clamped to
+// [0, 2^21 - 1]; a NaN coordinate quantises to 0.  Monotonic in p.
+inline uint32_t quantise(double p, double lo, double h) {
+  double s = std::floor((p - lo) / h + 0.5);
+  if (!(s >= 0)) s = 0;
+  if (s > 2097151.0) s = 2097151.0;
+  return (uint32_t)s;
+}
+
+// Locality order: the points quantised, (qx, qy, qz), sorted by bricks of tile[0] x tile[1] x
+// tile[2] (brick z, brick y, brick x, then z, y, x inside the brick), equal keys in index order.
+// tile = {huge, 1, 1} is plain (z, y, x) lexicographic order: x-runs are contiguous, so each
+// wavefront's gathers touch a few short runs.
+std::vector<int32_t> locality_order(int32_t n, const double* px, const double* py, const double* pz,
                                     const double h[3], const double lo[3], const int tile[3]) {
-  std::vector<uint32_t> q((size_t)n * 3);
+  const double* P[3] = {px, py, pz};
   const int nt = chunk_count(n);
-  std::vector<std::array<uint64_t, 3>> ext_of((size_t)nt, {1, 1, 1});  // per chunk
+  // Extents: quantisation is monotonic, so the largest quantised value along an axis is that
+  // of the largest coordinate (NaN coordinates, which quantise to 0, never compare larger).
+  std::vector<std::array<double, 3>> max_of((size_t)nt, {-INFINITY, -INFINITY, -INFINITY});
   parallel_chunks(n, [&](int w, int64_t b, int64_t e) {
-    auto& ext = ext_of[w];
-    for (int64_t i = b; i < e; i++) {
-      const double p[3] = {px[i], py[i], pz[i]};
-      for (int d = 0; d < 3; d++) {
-        double s = std::floor((p[d] - lo[d]) / h[d] + 0.5);
-        if (!(s >= 0)) s = 0;
-        if (s > 2097151.0) s = 2097151.0;
-        q[(size_t)i * 3 + d] = (uint32_t)s;
-        ext[d] = std::max<uint64_t>(ext[d], (uint64_t)s + 1);
-      }
+    for (int d = 0; d < 3; d++) {
+      double v = -INFINITY;
+      for (int64_t i = b; i < e; i++)
+        if (P[d][i] > v) v = P[d][i];
+      max_of[w][d] = v;
     }
   });
   uint64_t ext[3] = {1, 1, 1};
-  for (auto& e : ext_of)
-    for (int d = 0; d < 3; d++) ext[d] = std::max(ext[d], e[d]);
+  for (int d = 0; d < 3; d++) {
+    double v = -INFINITY;
+    for (auto& m : max_of)
+      if (m[d] > v) v = m[d];
+    ext[d] = (uint64_t)quantise(v, lo[d], h[d]) + 1;
+  }
   // brick (z, y, x), then (z, y, x) inside the brick; both fit 64 bits for extents < 2^21
   const uint64_t t[3] = {std::min<uint64_t>((uint64_t)tile[0], ext[0]),
                          std::min<uint64_t>((uint64_t)tile[1], ext[1]),
                          std::min<uint64_t>((uint64_t)tile[2], ext[2])};
   const uint64_t nb[3] = {(ext[0] + t[0] - 1) / t[0], (ext[1] + t[1] - 1) / t[1], (ext[2] + t[2] - 1) / t[2]};
+  const uint64_t vol = t[0] * t[1] * t[2];
+  // quantised values are below 2^21, so 32-bit division suffices; power-of-two tiles (the
+  // default) divide by shifting
+  const bool pow2 = ((t[0] & (t[0] - 1)) | (t[1] & (t[1] - 1)) | (t[2] & (t[2] - 1))) == 0;
+  int sh[3] = {0, 0, 0};
+  for (int d = 0; d < 3; d++)
+    while ((1ull << sh[d]) < t[d]) sh[d]++;
+  const uint32_t t32[3] = {(uint32_t)t[0], (uint32_t)t[1], (uint32_t)t[2]};
+  auto key_of = [&](int64_t i, uint64_t& brick, uint64_t& inner) {
+    const uint32_t qx = quantise(px[i], lo[0], h[0]), qy = quantise(py[i], lo[1], h[1]),
+                   qz = quantise(pz[i], lo[2], h[2]);
+    uint32_t b[3], r[3];
+    if (pow2) {
+      b[0] = qx >> sh[0], b[1] = qy >> sh[1], b[2] = qz >> sh[2];
+      r[0] = qx & (t32[0] - 1), r[1] = qy & (t32[1] - 1), r[2] = qz & (t32[2] - 1);
+    } else {
+      b[0] = qx / t32[0], b[1] = qy / t32[1], b[2] = qz / t32[2];
+      r[0] = qx - b[0] * t32[0], r[1] = qy - b[1] * t32[1], r[2] = qz - b[2] * t32[2];
+    }
+    brick = ((uint64_t)b[2] * nb[1] + b[1]) * nb[0] + b[0];
+    inner = ((uint64_t)r[2] * t[1] + r[1]) * t[0] + r[0];
+  };
   std::vector<int32_t> order((size_t)n);
   const long double span = (long double)(nb[0] * t[0]) * (long double)(nb[1] * t[1]) * (long double)(nb[2] * t[2]);
+  const int64_t step = ((int64_t)n + nt - 1) / nt;
+  // One stable LSD radix pass over the chunks on the digit (word >> shift) & (bins - 1): every
+  // chunk counts its digits, chunk-major prefix sums give each chunk its output range per
+  // digit, and each chunk scatters its words in order, so a pass produces what a serial stable
+  // pass produces.
+  auto radix_pass = [&](auto& src, auto& dst, auto&& digit, size_t bins, std::vector<int32_t>& count, auto&& move) {
+    count.assign(bins * nt, 0);
+    parallel_tasks(nt, [&](int w) {
+      int32_t* c = &count[bins * w];
+      const int64_t b = std::min<int64_t>(n, w * step), e = std::min<int64_t>(n, (w + 1) * step);
+      for (int64_t i = b; i < e; i++) c[digit(src[i])]++;
+    });
+    int32_t sum = 0;
+    for (size_t dgt = 0; dgt < bins; dgt++)
+      for (int w = 0; w < nt; w++) {
+        int32_t& c = count[bins * w + dgt];
+        const int32_t v = c;
+        c = sum;
+        sum += v;
+      }
+    parallel_tasks(nt, [&](int w) {
+      int32_t* c = &count[bins * w];
+      const int64_t b = std::min<int64_t>(n, w * step), e = std::min<int64_t>(n, (w + 1) * step);
+      for (int64_t i = b; i < e; i++) move(i, c[digit(src[i])]++);
+    });
+    src.swap(dst);
+  };
+  std::vector<int32_t> count;
+  if (span <= 4294967296.0L) {
+    // keys below 2^32: one word per point, (key << 32) | index, sorted on 8-bit digits of the
+    // key (few output streams per chunk, so the scatter stays in cache)
+    std::vector<uint64_t> a((size_t)n), a2((size_t)n);
+    std::vector<uint64_t> kmax_of((size_t)nt, 0);
+    parallel_chunks(n, [&](int w, int64_t b, int64_t e) {
+      uint64_t m = 0;
+      for (int64_t i = b; i < e; i++) {
+        uint64_t brick, inner;
+        key_of(i, brick, inner);
+        const uint64_t k = brick * vol + inner;
+        a[i] = (k << 32) | (uint64_t)i;
+        m = std::max(m, k);
+      }
+      kmax_of[w] = m;
+    });
+    const uint64_t kmax = *std::max_element(kmax_of.begin(), kmax_of.end());
+    for (int shift = 0; shift < 32 && (kmax >> shift) != 0; shift += 8) {
+      const int sh = 32 + shift;
+      radix_pass(a, a2, [sh](uint64_t x) { return (size_t)((x >> sh) & 255); }, 256, count,
+                 [&](int64_t i, int32_t dst) { a2[dst] = a[i]; });
+    }
+    parallel_ranges(n, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; i++) order[i] = (int32_t)(uint32_t)a[i];
+    });
+    return order;
+  }
   if (span < 9.2e18L) {
-    // one key per point, brick * (brick volume) + inner < span: a stable LSD radix sort on the
-    // bits the largest key uses keeps equal keys in index order.  Each pass is split over the
-    // chunks: every chunk counts its digits, chunk-major prefix sums give each chunk its
-    // output range per digit, and each chunk scatters its points in order, so the passes
-    // produce what a serial stable sort produces.
-    const uint64_t vol = t[0] * t[1] * t[2];
+    // one 64-bit key per point, brick * (brick volume) + inner < span, and the index beside it
     std::vector<uint64_t> k((size_t)n), k2((size_t)n);
     std::vector<int32_t> o2((size_t)n);
     std::vector<uint64_t> kmax_of((size_t)nt, 0);
     parallel_chunks(n, [&](int w, int64_t b, int64_t e) {
       uint64_t m = 0;
       for (int64_t i = b; i < e; i++) {
-        const uint64_t qx = q[(size_t)i * 3], qy = q[(size_t)i * 3 + 1], qz = q[(size_t)i * 3 + 2];
-        const uint64_t brick = ((qz / t[2]) * nb[1] + qy / t[1]) * nb[0] + qx / t[0];
-        const uint64_t inner = ((qz % t[2]) * t[1] + qy % t[1]) * t[0] + qx % t[0];
+        uint64_t brick, inner;
+        key_of(i, brick, inner);
         k[i] = brick * vol + inner;
         m = std::max(m, k[i]);
         order[i] = (int32_t)i;
@@ -434,44 +528,20 @@ std::vector<int32_t> locality_order(int32_t n, const std::vector<double>& px,
       kmax_of[w] = m;
     });
     const uint64_t kmax = *std::max_element(kmax_of.begin(), kmax_of.end());
-    constexpr int kDigit = 11;
-    constexpr size_t kBins = (size_t)1 << kDigit;
-    const int64_t step = ((int64_t)n + nt - 1) / nt;
-    std::vector<int32_t> count(kBins * nt);
-    for (int shift = 0; shift < 64 && (kmax >> shift) != 0; shift += kDigit) {
-      parallel_tasks(nt, [&](int w) {
-        int32_t* c = &count[kBins * w];
-        std::fill(c, c + kBins, 0);
-        const int64_t b = std::min<int64_t>(n, w * step), e = std::min<int64_t>(n, (w + 1) * step);
-        for (int64_t i = b; i < e; i++) c[(k[i] >> shift) & (kBins - 1)]++;
-      });
-      int32_t sum = 0;
-      for (size_t dgt = 0; dgt < kBins; dgt++)
-        for (int w = 0; w < nt; w++) {
-          int32_t& c = count[kBins * w + dgt];
-          const int32_t v = c;
-          c = sum;
-          sum += v;
-        }
-      parallel_tasks(nt, [&](int w) {
-        int32_t* c = &count[kBins * w];
-        const int64_t b = std::min<int64_t>(n, w * step), e = std::min<int64_t>(n, (w + 1) * step);
-        for (int64_t i = b; i < e; i++) {
-          const int32_t dst = c[(k[i] >> shift) & (kBins - 1)]++;
-          k2[dst] = k[i];
-          o2[dst] = order[i];
-        }
-      });
-      k.swap(k2);
+    for (int shift = 0; shift < 64 && (kmax >> shift) != 0; shift += 11) {
+      radix_pass(k, k2, [shift](uint64_t x) { return (size_t)((x >> shift) & 2047); }, 2048, count,
+                 [&](int64_t i, int32_t dst) {
+                   k2[dst] = k[i];
+                   o2[dst] = order[i];
+                 });
       order.swap(o2);
     }
     return order;
   }
   std::vector<std::pair<std::pair<uint64_t, uint64_t>, int32_t>> key((size_t)n);
   for (int32_t i = 0; i < n; i++) {
-    const uint64_t qx = q[(size_t)i * 3], qy = q[(size_t)i * 3 + 1], qz = q[(size_t)i * 3 + 2];
-    const uint64_t brick = ((qz / t[2]) * nb[1] + qy / t[1]) * nb[0] + qx / t[0];
-    const uint64_t inner = ((qz % t[2]) * t[1] + qy % t[1]) * t[0] + qx % t[0];
+    uint64_t brick, inner;
+    key_of(i, brick, inner);
     key[i] = {{brick, inner}, i};
   }
   std::sort(key.begin(), key.end());
@@ -589,6 +659,52 @@ struct Classifier {
     return last = (int32_t)n;
   }
 };
+
+// Material rows of n entities in the order given (row(c, m) fills the W values of entity c):
+// row_of[c] receives the entity's class and rows the class rows, numbered by first appearance
+// in that order; chunks are classified on their own threads and merged in chunk order.  False
+// when there are more than kMaxMaterials distinct rows.
+template <typename T, int W, typename F>
+bool classify_rows(int32_t n, F&& row, std::vector<int32_t>& row_of, std::vector<T>& rows) {
+  struct Chunk {
+    Classifier<T, W> mats{(size_t)rsg::kMaxMaterials};
+    bool overflow = false;
+    std::vector<int32_t> map;
+  };
+  std::vector<Chunk> chunks(chunk_count(n));
+  row_of.resize(n);
+  parallel_chunks(n, [&](int w, int64_t b, int64_t e) {
+    Chunk& ch = chunks[w];
+    T m[W];
+    for (int64_t c = b; c < e; c++) {
+      row((int32_t)c, m);
+      const int32_t k = ch.mats.add(m);
+      if (k < 0) {
+        ch.overflow = true;
+        return;
+      }
+      row_of[c] = k;
+    }
+  });
+  Classifier<T, W> all((size_t)rsg::kMaxMaterials);
+  for (auto& ch : chunks) {
+    if (ch.overflow) return false;
+    const size_t nm = ch.mats.table.size() / W;
+    ch.map.resize(nm);
+    for (size_t k = 0; k < nm; k++) {
+      T r[W];
+      memcpy(r, &ch.mats.table[k * W], sizeof(r));
+      const int32_t g = all.add(r);
+      if (g < 0) return false;
+      ch.map[k] = g;
+    }
+  }
+  parallel_chunks(n, [&](int w, int64_t b, int64_t e) {
+    for (int64_t c = b; c < e; c++) row_of[c] = chunks[w].map[row_of[c]];
+  });
+  rows = std::move(all.table);
+  return true;
+}
 
 // Keys of n entities from their material row m (W values) and spacings sp (3 values),
 // both in the handle's precision: material classes and per-axis spacing classes in order
@@ -1273,14 +1389,25 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   CREATE_PHASE("nodeprep");
   // locality order
   double lo[3], hmin[3];
-  for (int d = 0; d < 3; d++) {
-    lo[d] = *std::min_element(X[d], X[d] + nn);
-    hmin[d] = INFINITY;
-    for (int32_t c = 0; c < nc; c++) {
-      const double s = std::fabs(dcell[d][c]);
-      if (s > 0 && s < hmin[d]) hmin[d] = s;
+  {
+    // per chunk: the smallest positive cell spacing along each axis
+    std::vector<std::array<double, 3>> hmin_of((size_t)chunk_count(nc), {INFINITY, INFINITY, INFINITY});
+    parallel_chunks(nc, [&](int w, int64_t b, int64_t e) {
+      for (int d = 0; d < 3; d++) {
+        double m = INFINITY;
+        for (int64_t c = b; c < e; c++) {
+          const double s = std::fabs(dcell[d][c]);
+          if (s > 0 && s < m) m = s;
+        }
+        hmin_of[w][d] = m;
+      }
+    });
+    for (int d = 0; d < 3; d++) {
+      lo[d] = *std::min_element(X[d], X[d] + nn);
+      hmin[d] = INFINITY;
+      for (auto& m : hmin_of) hmin[d] = std::min(hmin[d], m[d]);
+      if (!std::isfinite(hmin[d])) hmin[d] = 1.0;
     }
-    if (!std::isfinite(hmin[d])) hmin[d] = 1.0;
   }
   {
     std::vector<double> cx(nc), cy(nc), cz(nc);
@@ -1295,27 +1422,30 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
       }
     });
     double clo[3] = {lo[0] + 0.5 * hmin[0], lo[1] + 0.5 * hmin[1], lo[2] + 0.5 * hmin[2]};
-    std::vector<double> nx(X[0], X[0] + nn), ny(X[1], X[1] + nn), nz(X[2], X[2] + nn);
     bool cells_ok = true, nodes_ok = true;
     std::thread cell_sort([&]() {  // the two orders are independent
       try {
-        h->cell_of = locality_order(nc, cx, cy, cz, hmin, clo, h->tile);
+        h->cell_of = locality_order(nc, cx.data(), cy.data(), cz.data(), hmin, clo, h->tile);
       } catch (const std::bad_alloc&) {
         cells_ok = false;
       }
     });
     try {
-      h->node_of = locality_order(nn, nx, ny, nz, hmin, lo, h->tile);
+      h->node_of = locality_order(nn, X[0], X[1], X[2], hmin, lo, h->tile);
     } catch (const std::bad_alloc&) {
       nodes_ok = false;
     }
     cell_sort.join();
     if (!cells_ok || !nodes_ok) return fail(RSG_ERR_NOMEM, "host allocation failed");
   }
-  h->cell_at.assign(nc, -1);
-  h->node_at.assign(nn, -1);
-  for (int32_t i = 0; i < nc; i++) h->cell_at[h->cell_of[i]] = i;
-  for (int32_t i = 0; i < nn; i++) h->node_at[h->node_of[i]] = i;
+  h->cell_at.resize(nc);
+  h->node_at.resize(nn);
+  parallel_ranges(nc, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) h->cell_at[h->cell_of[i]] = (int32_t)i;
+  });
+  parallel_ranges(nn, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) h->node_at[h->node_of[i]] = (int32_t)i;
+  });
 
   CREATE_PHASE("order");
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -1447,15 +1577,38 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
       return precision == RSG_FP64 ? cij : h->dt * cij;
     };
     if (h->classes) {  // material: the 21 C; spacings as the per-cell arrays hold them
-      auto entity = [&](int32_t i, auto& m, auto& sp) {
+      // The rows are classified in the caller's cell order first, where each of the 21 fields
+      // is read sequentially; an internal cell then copies its row from the small class
+      // table.  make_keys numbers the classes by first appearance in internal order, as it
+      // would from the fields themselves.
+      std::vector<int32_t> row_of;
+      std::vector<double> rows_d;
+      std::vector<float> rows_f;
+      auto caller_row = [&](int32_t c, auto& m) {
         using T = std::remove_reference_t<decltype(m[0])>;
-        for (int j = 0; j < 21; j++) m[j] = (T)c_entry(i, j);
-        for (int d = 0; d < 3; d++) sp[d] = (T)dc[d][i];
+        for (int j = 0; j < 21; j++) {
+          const double cij = field_at_cell(model->cij[j], h->cells.data(), c);
+          m[j] = (T)(precision == RSG_FP64 ? cij : h->dt * cij);
+        }
       };
-      rc = precision == RSG_FP64
-               ? make_keys<double, 21, rsg::kMatRow>(h.get(), nc, entity, h->ckey_h, &h->cmat, &h->caxis, &h->n_cmat)
-               : make_keys<float, 21, rsg::kMatRow>(h.get(), nc, entity, h->ckey_h, &h->cmat, &h->caxis, &h->n_cmat);
-      if (rc) return rc;
+      const bool few = precision == RSG_FP64 ? classify_rows<double, 21>(nc, caller_row, row_of, rows_d)
+                                             : classify_rows<float, 21>(nc, caller_row, row_of, rows_f);
+      if (few) {  // else more materials than the tables hold: per-cell arrays
+        auto entity = [&](int32_t i, auto& m, auto& sp) {
+          using T = std::remove_reference_t<decltype(m[0])>;
+          const T* r;
+          if constexpr (std::is_same<T, double>::value)
+            r = rows_d.data() + (size_t)row_of[h->cell_of[i]] * 21;
+          else
+            r = rows_f.data() + (size_t)row_of[h->cell_of[i]] * 21;
+          for (int j = 0; j < 21; j++) m[j] = r[j];
+          for (int d = 0; d < 3; d++) sp[d] = (T)dc[d][i];
+        };
+        rc = precision == RSG_FP64
+                 ? make_keys<double, 21, rsg::kMatRow>(h.get(), nc, entity, h->ckey_h, &h->cmat, &h->caxis, &h->n_cmat)
+                 : make_keys<float, 21, rsg::kMatRow>(h.get(), nc, entity, h->ckey_h, &h->cmat, &h->caxis, &h->n_cmat);
+        if (rc) return rc;
+      }
     }
     if (h->n_cmat < 0) {
       std::vector<double> C((size_t)21 * ncp, 0.0);
