@@ -70,6 +70,9 @@ struct VelocityArgs {
   const int32_t* ulist;    // cells staged by each block
   const uint16_t* lconn;   // [n][8] block-local cell index by slot, 0xFFFF = none
   int32_t max_u;           // largest unique set of a block (LDS rows hold max_u + 1)
+  int32_t uchunk;          // 1: ulist holds chunk words, start << 2 | (len - 1): runs of up to 4
+                           // consecutive cells, each taking 4 LDS slots (slot 4c + k holds
+                           // start + k for k < len); max_u and the links count slots
   // special nodes (sources and/or sensors)
   const T* sp_idt;         // [nsp] real dt/rho_n
   const int32_t* sp_off;   // [nsp][4] term ranges for directions 0,1,2

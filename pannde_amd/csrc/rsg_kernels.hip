@@ -233,11 +233,52 @@ __device__ __forceinline__ uint32_t load_lkey(const uint32_t* __restrict__ lkey,
 // thread owns at most U entries and all their loads are issued before any wait (two
 // dependent round trips in all: the list, then the values); U = 0 is the general loop.
 // zero >= 0 also stores 0 at entry `zero` of every row (the target of missing links; it
-// lies past every block's set).
-template <typename T, int NCOMP, int U>
+// lies past every block's set).  With CHUNKS and `chunked` the list holds one word per 4 LDS
+// slots, start << 2 | (len - 1), and slot 4c + k takes entity start + k when k < len (the
+// other slots of the chunk stay unwritten: no link points at them).
+template <typename T, int NCOMP, int U, bool CHUNKS = false>
 __device__ __forceinline__ void stage(T* lds, int ld, const int32_t* __restrict__ u_off,
                                       const int32_t* __restrict__ ulist, int b,
-                                      const T* const (&src)[NCOMP], int zero) {
+                                      const T* const (&src)[NCOMP], int zero, bool chunked = false) {
+  if constexpr (CHUNKS) {
+    if (chunked) {
+      const int c0 = u_off[b], nu = 4 * (u_off[b + 1] - c0);
+      if constexpr (U > 0) {
+        int g[U];
+        bool on[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int i = threadIdx.x + u * blockDim.x;
+          const int w = STREAM_LOAD(ulist + c0 + (i < nu ? (i >> 2) : 0));
+          on[u] = i < nu && (i & 3) <= (w & 3);
+          g[u] = on[u] ? (w >> 2) + (i & 3) : 0;
+        }
+        T v[U][NCOMP];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int k = 0; k < NCOMP; k++) v[u][k] = on[u] ? src[k][g[u]] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int i = threadIdx.x + u * blockDim.x;
+          if (on[u])
+#pragma unroll
+            for (int k = 0; k < NCOMP; k++) lds[k * ld + i] = v[u][k];
+        }
+      } else {
+        for (int i = threadIdx.x; i < nu; i += blockDim.x) {
+          const int w = STREAM_LOAD(ulist + c0 + (i >> 2));
+          if ((i & 3) > (w & 3)) continue;
+          const int g = (w >> 2) + (i & 3);
+#pragma unroll
+          for (int k = 0; k < NCOMP; k++) lds[k * ld + i] = src[k][g];
+        }
+      }
+      if (zero >= 0 && (int)threadIdx.x < NCOMP) lds[threadIdx.x * ld + zero] = T(0);
+      __syncthreads();
+      return;
+    }
+  }
   const int u0 = u_off[b], nu = u_off[b + 1] - u0;
   if constexpr (U > 0) {
     int g[U];
@@ -403,7 +444,7 @@ __device__ __forceinline__ void velocity_body(const VelocityArgs<T>& a, int b, u
     const T* const src[6] = {a.S0, a.S1, a.S2, a.S3, a.S4, a.S5};
     // rows of max_u + 1 entries: entry max_u is the zero that missing cells read
     const int ld = a.max_u + 1;
-    stage<T, 6, U>(lds, ld, a.u_off, a.ulist, b, src, a.max_u);
+    stage<T, 6, U, true>(lds, ld, a.u_off, a.ulist, b, src, a.max_u, a.uchunk != 0);
     S0 = lds;
     S1 = lds + ld;
     S2 = lds + 2 * ld;
