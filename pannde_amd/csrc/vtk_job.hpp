@@ -219,9 +219,10 @@ class ByteSource {
   }
   void read(size_t n, std::vector<unsigned char>& out) {
     out.clear();
+    // sizes come from the file: check them against what is left before allocating
+    if (n > (len_ - pos_) / (raw_ ? 1 : 4) * (raw_ ? 1 : 3) + (raw_ ? 0 : 3)) throw Error("binary data ends early");
     out.reserve(n);
     if (raw_) {
-      if (n > len_ - pos_) throw Error("binary data ends early");
       out.insert(out.end(), raw_ + pos_, raw_ + pos_ + n);
       pos_ += n;
       return;
@@ -392,6 +393,8 @@ class Decoder {
     for (uint64_t b = 0; b < nb; b++) {
       src.read((size_t)header(sizes.data() + b * hb_), comp);
       const uint64_t want = (b + 1 == nb && last) ? last : bs;
+      // a deflate stream expands at most about 1032:1; a larger claim is a corrupt header
+      if (want > (uint64_t)comp.size() * 1032 + 64) throw Error("compressed block size is not valid");
       const size_t at = out.size();
       out.resize(at + want);
       uLongf got = (uLongf)want;

@@ -244,3 +244,27 @@ def test_job_errors(tmp_path):
     assert r.returncode == 1 and "NWaterColumns" in r.stdout
     r = subprocess.run([DRIVER, "-f", str(job)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert r.returncode == 1 and "-o out" in r.stdout
+
+
+def test_corrupt_sizes_fail_cleanly(dumper, tmp_path):
+    # sizes in the file that exceed the data (a huge byte count, a zlib block claiming far more
+    # than its compressed bytes can hold) are errors, not allocations
+    from test_vtkio import ASCII_JOB, replace_array
+
+    head = struct.pack("<Q", 1 << 40)
+    el = '<DataArray type="Float64" Name="dt" format="binary">%s</DataArray>' % base64.b64encode(
+        head + struct.pack("<d", 1.0)).decode()
+    xml = ASCII_JOB.replace('version="0.1" byte_order="LittleEndian"',
+                            'version="1.0" byte_order="LittleEndian" header_type="UInt64"')
+    p = tmp_path / "huge.vtu"
+    p.write_text(replace_array(xml, 'Name="dt"', el))
+    with pytest.raises(vtkio.VtkError, match="ends early"):
+        cpp_read(dumper, p)
+    comp = zlib.compress(struct.pack("<2d", 1.0, 2.0))
+    head = struct.pack("<4Q", 1, 1 << 40, 1 << 40, len(comp))
+    el = '<DataArray type="Float64" Name="dt" format="binary">%s</DataArray>' % (
+        base64.b64encode(head) + base64.b64encode(comp)).decode()
+    xml = xml.replace('header_type="UInt64"', 'header_type="UInt64" compressor="vtkZLibDataCompressor"')
+    p.write_text(replace_array(xml, 'Name="dt"', el))
+    with pytest.raises(vtkio.VtkError, match="not valid"):
+        cpp_read(dumper, p)
