@@ -144,29 +144,87 @@ struct Channel {
   Signal sig;
 };
 
-// A device buffer owned by its handle: released on reallocation and with the handle.
+// Device memory of the handles comes from a stream-ordered pool of the library's own, one per
+// device.  hipFree waits for every stream of the device before it frees, so a process that
+// destroys handles while batches keep other streams busy (a server) would wait about one K-step
+// block per buffer; hipFreeAsync on the handle's own stream, idle by then, returns the memory
+// to the pool without that wait, and later handles reuse it.  The pool keeps what is freed
+// (its release threshold is unbounded) until the process ends.
+hipError_t device_pool(hipMemPool_t* out) {
+  static std::mutex mu;
+  static std::vector<std::pair<int, hipMemPool_t>> pools;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  for (auto& dp : pools)
+    if (dp.first == dev) {
+      *out = dp.second;
+      return hipSuccess;
+    }
+  hipMemPoolProps props;
+  memset(&props, 0, sizeof(props));
+  props.allocType = hipMemAllocationTypePinned;
+  props.handleTypes = hipMemHandleTypeNone;
+  props.location.type = hipMemLocationTypeDevice;
+  props.location.id = dev;
+  hipMemPool_t pool = nullptr;
+  if ((e = hipMemPoolCreate(&pool, &props)) != hipSuccess) return e;
+  uint64_t keep = UINT64_MAX;
+  if ((e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep)) != hipSuccess) return e;
+  pools.emplace_back(dev, pool);
+  *out = pool;
+  return hipSuccess;
+}
+
+// `bytes` of device memory from the pool, ordered on stream s (of the current device): work
+// queued on s may use it at once, the host and other streams once s has been synchronized.
+hipError_t device_alloc(void** p, size_t bytes, hipStream_t s) {
+  hipMemPool_t pool = nullptr;
+  hipError_t e = device_pool(&pool);
+  if (e == hipSuccess) e = hipMallocFromPoolAsync(p, bytes, pool, s);
+  return e;
+}
+
+// Returns pool memory once the work queued on s so far has run; the caller guarantees that no
+// other stream still uses it.
+void device_free(void* p, hipStream_t s) { (void)hipFreeAsync(p, s); }
+
+// A copy ordered on s (a non-blocking stream) that has finished when this returns.  hipMemcpy
+// runs on the null stream and waits for the work queued on every stream, so while batches keep
+// other streams busy a handle's uploads would wait behind them.
+hipError_t copy_on(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
+}
+
+// A device buffer owned by its handle, from the device pool on the handle's stream: released
+// on reallocation, and by the handle before it destroys that stream.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  hipStream_t s = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
-  int alloc(size_t count) {
+  int alloc(size_t count, hipStream_t stream) {
     release();
     n = count;
     if (count == 0) return RSG_OK;
-    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    hipError_t e = device_alloc((void**)&p, count * sizeof(T), stream);
     if (e != hipSuccess) {
       p = nullptr;
       n = 0;
-      return fail(RSG_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+      return fail(RSG_ERR_NOMEM, "device allocation (%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
     }
+    s = stream;
     return RSG_OK;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) device_free(p, s);
     p = nullptr;
     n = 0;
   }
@@ -237,6 +295,10 @@ struct rsg_instance {
   uint32_t* s_lkey = nullptr;
   std::vector<uint16_t> s_lconn_h;
   int32_t s_maxu = 0;
+  // entities whose key on the device carries the special flag (internal ids, ascending): the
+  // next table build restores their plain keys unless they stay special
+  std::vector<int32_t> v_flagged, s_flagged;
+  bool incidence_exact = true;  // node_cells holds every cell of every node (no slot claimed twice)
   double* node_m = nullptr;    // node masses rho_n*dxn*dyn*dzn (energy)
   double* cell_vol = nullptr;  // cell volumes dxc*dyc*dzc (energy)
 
@@ -255,6 +317,7 @@ struct rsg_instance {
   DevBuf<int32_t> cell_terms;
   DevBuf<double> amp[2];
   double* stage[2] = {nullptr, nullptr};  // pinned host staging
+  size_t stage_len[2] = {0, 0};            // doubles each staging buffer holds
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   bool stage_busy[2] = {false, false};
   int cur = 0;
@@ -277,13 +340,19 @@ struct rsg_instance {
 
   std::mutex alloc_mu;  // set-up allocates from two threads
 
+  // host sources of the copies queued on `stream` since it was last synchronized; a set-up
+  // queues all its uploads and waits once (finish_uploads)
+  std::mutex hold_mu;
+  std::vector<std::shared_ptr<const void>> held;
+
+  // device memory from the pool on the handle's stream, released with the handle
   int dalloc(void** p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) bytes = 8;
-    hipError_t e = hipMalloc(p, bytes);
+    hipError_t e = device_alloc(p, bytes, stream);
     if (e != hipSuccess) {
       *p = nullptr;
-      return fail(RSG_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+      return fail(RSG_ERR_NOMEM, "device allocation (%zu bytes): %s", bytes, hipGetErrorString(e));
     }
     std::lock_guard<std::mutex> lock(alloc_mu);
     allocations.push_back(*p);
@@ -292,19 +361,23 @@ struct rsg_instance {
   }
 
   ~rsg_instance() {
-    if (stream) (void)hipStreamSynchronize(stream);
-    for (void* p : allocations) (void)hipFree(p);
+    // every buffer is freed on the handle's stream, in order behind its own work; a handle
+    // in a batch cannot be destroyed, and leaving the batch waited for the lane's work on it
+    for (void* p : allocations) device_free(p, stream);
     sp_node_val.release();
     sp_node_off.release();
     sp_node_sens.release();
     node_terms.release();
     sp_cell_val.release();
+    sp_node_key.release();
+    sp_cell_key.release();
     sp_cell_off.release();
     cell_terms.release();
     amp[0].release();
     amp[1].release();
     trace.release();
     partial.release();
+    if (stream) (void)hipStreamSynchronize(stream);
     for (int i = 0; i < 2; i++) {
       if (stage[i]) (void)hipHostFree(stage[i]);
       if (stage_ev[i]) (void)hipEventDestroy(stage_ev[i]);
@@ -348,11 +421,25 @@ __attribute__((always_inline)) inline double field_at_cell(const rsg_field_desc&
   return f.type == RSG_FIELD_CONSTANT ? f.data[0] : f.data[c];
 }
 
-// Set-up loops over n entities run in up to 8 contiguous chunks of at least 64 Ki entities.
+// Threads of one set-up: 8, or PANNDE_RSG_SETUP_THREADS (1..64), read once per process.  A
+// process that sets up many handles at once (a server's preparation threads) may spend less
+// CPU in total with fewer threads per set-up.  Results do not depend on it.
+int setup_threads() {
+  static const int t = [] {
+    const char* e = std::getenv("PANNDE_RSG_SETUP_THREADS");
+    const int v = e ? std::atoi(e) : 0;
+    return v >= 1 && v <= 64 ? v : 8;
+  }();
+  return t;
+}
+
+// Set-up loops over n entities run in up to setup_threads() contiguous chunks of at least
+// 64 Ki entities.
 int chunk_count(int64_t n) {
   constexpr int64_t kMinChunk = 1 << 16;
   const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
-  return (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(8, hw), (n + kMinChunk - 1) / kMinChunk));
+  return (int)std::max<int64_t>(
+      1, std::min<int64_t>(std::min<int64_t>(setup_threads(), hw), (n + kMinChunk - 1) / kMinChunk));
 }
 
 // Runs fn(w) for w in [0, t), one thread each (the caller takes w = 0); an exception of any
@@ -562,7 +649,7 @@ struct LdsPlan {
   int32_t max_u = 0;
 };
 
-bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other, int block, LdsPlan& p,
+bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int block, LdsPlan& p,
                     bool chunked = false) {
   const int32_t nb = (n + block - 1) / block;
   p.u_off.assign((size_t)nb + 1, 0);
@@ -571,8 +658,9 @@ bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other
   p.max_u = 0;
   // contiguous ranges of blocks on their own threads, each with its own tag array: the block
   // that last saw an entity (high 32 bits) and the entity's index in that block's set (low
-  // bits), so finding and numbering a link touches one word.  The per-range lists are joined
-  // in block order.
+  // bits), so finding and numbering a link touches one word.  A range's tag array covers only
+  // the window of ids its links reach (in locality order a narrow one).  The per-range lists
+  // are joined in block order.
   struct Range {
     std::vector<int32_t> ulist;
     int32_t max_u = 0;
@@ -583,7 +671,17 @@ bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other
   parallel_tasks((int)nr, [&](int w) {
     Range& r = ranges[w];
     const int32_t b0 = (int32_t)std::min<int64_t>(nb, w * step), b1 = (int32_t)std::min<int64_t>(nb, (w + 1) * step);
-    std::vector<uint64_t> tag((size_t)n_other, ~0ull);
+    const size_t e0 = (size_t)b0 * block * 8, e1 = (size_t)std::min<int64_t>(n, (int64_t)b1 * block) * 8;
+    int32_t w_lo = INT32_MAX, w_hi = -1;
+    for (size_t i = e0; i < e1; i++) {
+      const int32_t g = conn[i];
+      if (g >= 0) {
+        w_lo = std::min(w_lo, g);
+        w_hi = std::max(w_hi, g);
+      }
+    }
+    if (w_hi < 0) w_lo = w_hi = 0;  // no links: every set is empty
+    std::vector<uint64_t> tag((size_t)(w_hi - w_lo) + 1, ~0ull);
     std::vector<int32_t> ids;
     ids.reserve((size_t)block * 8);
     r.ulist.reserve((size_t)(b1 - b0) * block * 2);
@@ -591,14 +689,24 @@ bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other
       const int32_t lo = b * block, hi = std::min(n, lo + block);
       const uint64_t stamp = (uint64_t)(uint32_t)b << 32;
       ids.clear();
+      int32_t s_lo = INT32_MAX, s_hi = -1;
       for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++) {
         const int32_t g = conn[i];
-        if (g >= 0 && (tag[g] & ~0xFFFFFFFFull) != stamp) {
-          tag[g] = stamp;
+        if (g >= 0 && (tag[g - w_lo] & ~0xFFFFFFFFull) != stamp) {
+          tag[g - w_lo] = stamp;
           ids.push_back(g);
+          s_lo = std::min(s_lo, g);
+          s_hi = std::max(s_hi, g);
         }
       }
-      std::sort(ids.begin(), ids.end());
+      // the set in increasing order: read off the tags when it is dense in its span, else sorted
+      if (!ids.empty() && (int64_t)s_hi - s_lo < 16 * (int64_t)ids.size()) {
+        size_t k = 0;
+        for (int32_t g = s_lo; g <= s_hi; g++)
+          if ((tag[g - w_lo] & ~0xFFFFFFFFull) == stamp) ids[k++] = g;
+      } else {
+        std::sort(ids.begin(), ids.end());
+      }
       int32_t slots = 0, words = 0;
       if (chunked) {
         for (size_t k = 0; k < ids.size();) {
@@ -608,14 +716,14 @@ bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other
           }
           int32_t len = 1;
           while (len < 4 && k + len < ids.size() && ids[k + len] == ids[k] + len) len++;
-          for (int32_t o = 0; o < len; o++) tag[ids[k + o]] = stamp | (uint64_t)(slots + o);
+          for (int32_t o = 0; o < len; o++) tag[ids[k + o] - w_lo] = stamp | (uint64_t)(slots + o);
           r.ulist.push_back((ids[k] << 2) | (len - 1));
           slots += 4;
           words++;
           k += len;
         }
       } else {
-        for (size_t k = 0; k < ids.size(); k++) tag[ids[k]] = stamp | k;
+        for (size_t k = 0; k < ids.size(); k++) tag[ids[k] - w_lo] = stamp | k;
         r.ulist.insert(r.ulist.end(), ids.begin(), ids.end());
         slots = words = (int32_t)ids.size();
       }
@@ -624,7 +732,7 @@ bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other
         return;
       }
       for (size_t i = (size_t)lo * 8; i < (size_t)hi * 8; i++)
-        if (conn[i] >= 0) p.lconn[i] = (uint16_t)(tag[conn[i]] & 0xFFFFFFFFull);
+        if (conn[i] >= 0) p.lconn[i] = (uint16_t)(tag[conn[i] - w_lo] & 0xFFFFFFFFull);
       p.u_off[b + 1] = words;  // list lengths for now, offsets below
       r.max_u = std::max(r.max_u, slots);
     }
@@ -641,20 +749,41 @@ bool build_lds_plan(const std::vector<int32_t>& conn, int32_t n, int32_t n_other
   return true;
 }
 
+// Queues a host-to-device copy of src (moved in and held until finish_uploads) on the
+// handle's stream.  A synchronous copy per array would wait, each time, for whatever the
+// device's busy hardware queues hold ahead of it.
 template <typename T>
-int upload(rsg_instance* h, void** dst, const std::vector<T>& src) {
-  int rc = h->dalloc(dst, src.size() * sizeof(T));
-  if (rc) return rc;
-  HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+int queue_upload(rsg_instance* h, void* dst, std::vector<T>&& src) {
+  if (src.empty()) return RSG_OK;
+  auto keep = std::make_shared<std::vector<T>>(std::move(src));
+  HIP_TRY(hipMemcpyAsync(dst, keep->data(), keep->size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  std::lock_guard<std::mutex> lock(h->hold_mu);
+  h->held.push_back(std::move(keep));
   return RSG_OK;
 }
 
+// Waits for the handle's queued copies and releases their sources.
+int finish_uploads(rsg_instance* h) {
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  std::lock_guard<std::mutex> lock(h->hold_mu);
+  h->held.clear();
+  return RSG_OK;
+}
+
+// Allocates a device array for src (moved in) and queues its upload.
+template <typename T>
+int upload(rsg_instance* h, void** dst, std::vector<T>&& src) {
+  int rc = h->dalloc(dst, src.size() * sizeof(T));
+  if (rc) return rc;
+  return queue_upload(h, *dst, std::move(src));
+}
+
 // Converts FP64 values to the handle's precision and uploads them.
-int upload_real(rsg_instance* h, void** dst, const std::vector<double>& src) {
-  if (h->precision == RSG_FP64) return upload(h, dst, src);
+int upload_real(rsg_instance* h, void** dst, std::vector<double> src) {
+  if (h->precision == RSG_FP64) return upload(h, dst, std::move(src));
   std::vector<float> f(src.size());
   for (size_t i = 0; i < src.size(); i++) f[i] = (float)src[i];
-  return upload(h, dst, f);
+  return upload(h, dst, std::move(f));
 }
 
 // Assigns rows of W values (in the handle's precision) to classes of equal rows, in
@@ -668,9 +797,16 @@ struct Classifier {
   explicit Classifier(size_t m) : max_classes(m) {}
   int32_t add(const T (&row)[W]) {
     if (last >= 0 && memcmp(row, &table[(size_t)last * W], sizeof(row)) == 0) return last;
-    uint64_t hsh = 1469598103934665603ull;  // FNV-1a over the row's bytes
+    // the row's bytes hashed 8 at a time (multiply and fold); equal rows are found by comparing
+    // the rows themselves, so the hash only picks the bucket
+    uint64_t hsh = 0x9E3779B97F4A7C15ull;
     const unsigned char* b = reinterpret_cast<const unsigned char*>(row);
-    for (size_t i = 0; i < sizeof(row); i++) hsh = (hsh ^ b[i]) * 1099511628211ull;
+    for (size_t i = 0; i < sizeof(row); i += 8) {
+      uint64_t wd = 0;
+      memcpy(&wd, b + i, std::min<size_t>(8, sizeof(row) - i));
+      hsh = (hsh ^ wd) * 0xFF51AFD7ED558CCDull;
+      hsh ^= hsh >> 29;
+    }
     auto& list = index[hsh];
     for (int32_t c : list)
       if (memcmp(row, &table[(size_t)c * W], sizeof(row)) == 0) return last = c;
@@ -839,8 +975,8 @@ int make_keys(rsg_instance* h, int32_t n, F&& entity, std::vector<uint32_t>& key
   for (int d = 0; d < 3; d++)
     for (size_t k = 0; k < axv[d].size(); k++) at[(size_t)d * rsg::kAxisClasses + k] = axv[d][k];
   int rc;
-  if ((rc = upload(h, mat_table, mt))) return rc;
-  if ((rc = upload(h, axis_table, at))) return rc;
+  if ((rc = upload(h, mat_table, std::move(mt)))) return rc;
+  if ((rc = upload(h, axis_table, std::move(at)))) return rc;
   *materials = (int32_t)nm;
   return RSG_OK;
 }
@@ -852,154 +988,203 @@ int copy_boxed(rsg_instance* h, void* dst, const std::vector<double>& src,
   if (h->precision == RSG_FP64) {
     std::vector<double> d(src);
     for (auto& b : boxes) d[b.first] = rsg::box_f64(b.second);
-    HIP_TRY(hipMemcpy(dst, d.data(), d.size() * 8, hipMemcpyHostToDevice));
+    return queue_upload(h, dst, std::move(d));
   } else {
     std::vector<float> f(src.size());
     for (size_t i = 0; i < src.size(); i++) f[i] = (float)src[i];
     for (auto& b : boxes) f[b.first] = rsg::box_f32(b.second);
-    HIP_TRY(hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    return queue_upload(h, dst, std::move(f));
   }
   return RSG_OK;
 }
 
 // Packs 8 16-bit local links (0xFFFF = none) and the key of every entity into the 16-byte
 // records the packed kernels read: 10 bits per link, 3 links per word, the key in word 3.
-int upload_packed(uint32_t* dst, const std::vector<uint16_t>& lconn, const std::vector<uint32_t>& keys) {
+inline void pack_links(const uint16_t* l8, uint32_t* w3) {
+  w3[0] = w3[1] = w3[2] = 0;
+  for (int k = 0; k < 8; k++) {
+    const uint32_t v = l8[k] == 0xFFFF ? rsg::kNoLink : (uint32_t)l8[k];
+    w3[k / 3] |= v << (10 * (k % 3));
+  }
+}
+
+int upload_packed(rsg_instance* h, uint32_t* dst, const std::vector<uint16_t>& lconn,
+                  const std::vector<uint32_t>& keys) {
   const size_t n = keys.size();
   std::vector<uint32_t> rec(n * 4);
   for (size_t i = 0; i < n; i++) {
-    uint32_t w[3] = {0, 0, 0};
-    for (int k = 0; k < 8; k++) {
-      const uint16_t l = lconn[i * 8 + k];
-      const uint32_t v = l == 0xFFFF ? rsg::kNoLink : (uint32_t)l;
-      w[k / 3] |= v << (10 * (k % 3));
-    }
-    rec[i * 4] = w[0];
-    rec[i * 4 + 1] = w[1];
-    rec[i * 4 + 2] = w[2];
+    pack_links(&lconn[i * 8], &rec[i * 4]);
     rec[i * 4 + 3] = keys[i];
   }
-  HIP_TRY(hipMemcpy(dst, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
+  return queue_upload(h, dst, std::move(rec));
+}
+
+// Rewrites on the device the keys whose special flag changes: those of the entities flagged by
+// the last table build (`flagged`, which then lists the entities flagged now) and those of the
+// special entities now (`boxes`: internal id and special index, ids ascending).  `packed`: the
+// 16-byte records (links from lconn, the key in word 3); else the 4-byte keys.  Set-up uploads
+// every entity's plain key, so a build touches only the few special entities.
+int patch_keys(rsg_instance* h, uint32_t* dst, bool packed, const std::vector<uint16_t>& lconn,
+               const std::vector<uint32_t>& base, const std::vector<std::pair<int32_t, int32_t>>& boxes,
+               std::vector<int32_t>& flagged) {
+  std::vector<int32_t> ids(flagged);
+  for (auto& b : boxes) ids.push_back(b.first);
+  std::sort(ids.begin(), ids.end());
+  ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+  const size_t W = packed ? 4 : 1;
+  auto rec = std::make_shared<std::vector<uint32_t>>(ids.size() * W);
+  size_t q = 0;
+  for (size_t a = 0; a < ids.size(); a++) {
+    const int32_t i = ids[a];
+    while (q < boxes.size() && boxes[q].first < i) q++;
+    const bool special = q < boxes.size() && boxes[q].first == i;
+    const uint32_t key = special ? rsg::kSpecialKey | (uint32_t)boxes[q].second : base[i];
+    if (packed) {
+      pack_links(&lconn[(size_t)i * 8], &(*rec)[a * 4]);
+      (*rec)[a * 4 + 3] = key;
+    } else {
+      (*rec)[a] = key;
+    }
+  }
+  flagged.clear();
+  for (auto& b : boxes) flagged.push_back(b.first);
+  for (size_t a = 0; a < ids.size();) {  // one copy per run of consecutive ids
+    size_t e = a + 1;
+    while (e < ids.size() && ids[e] == ids[e - 1] + 1) e++;
+    HIP_TRY(hipMemcpyAsync(dst + (size_t)ids[a] * W, rec->data() + a * W, (e - a) * W * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, h->stream));
+    a = e;
+  }
+  std::lock_guard<std::mutex> lock(h->hold_mu);
+  h->held.push_back(std::move(rec));
   return RSG_OK;
 }
 
 // Builds the special-node / special-cell tables from the channels and sensors, and
 // NaN-boxes the idt / dcx entries of the special entities on the device.
 int build_tables(rsg_instance* h) {
-  const int32_t nn = h->nn, nc = h->nc;
-  // force channels covering each cell (caller numbering), in channel order
-  std::vector<std::vector<int32_t>> cell_ch, node_ch;
-  bool any_force = false, any_moment = false;
+  const int32_t nn = h->nn;
+  // (entity, channel) for every support entry, ordered by entity then channel: the force
+  // channels covering a cell and the moment channels covering a node (caller numbering) are a
+  // run of the list, in channel order
+  std::vector<std::pair<int32_t, int32_t>> cell_k, node_k;
   for (size_t k = 0; k < h->channels.size(); k++) {
     const Channel& ch = h->channels[k];
-    if (ch.kind == RSG_FORCE) {
-      if (!any_force) cell_ch.assign(nc, {});
-      any_force = true;
-      for (int32_t c : ch.support) cell_ch[c].push_back((int32_t)k);
-    } else {
-      if (!any_moment) node_ch.assign(nn, {});
-      any_moment = true;
-      for (int32_t n : ch.support) node_ch[n].push_back((int32_t)k);
-    }
+    auto& list = ch.kind == RSG_FORCE ? cell_k : node_k;
+    for (int32_t e : ch.support) list.push_back({e, (int32_t)k});
   }
-  // special nodes: touched by a force channel, or a sensor (internal order)
-  std::vector<int32_t> sens_slot(nn, -1);
-  for (size_t s = 0; s < h->sensors.size(); s++) sens_slot[h->sensors[s]] = (int32_t)s;
-  std::vector<char> is_sp(nn, 0);
-  if (any_force)
-    for (int32_t c = 0; c < nc; c++)
-      if (!cell_ch[c].empty())
-        for (int v = 0; v < 8; v++) is_sp[h->cells[(size_t)c * 8 + v]] = 1;
-  for (int32_t s : h->sensors) is_sp[s] = 1;
+  std::sort(cell_k.begin(), cell_k.end());
+  std::sort(node_k.begin(), node_k.end());
+  using Run = std::pair<std::vector<std::pair<int32_t, int32_t>>::const_iterator,
+                        std::vector<std::pair<int32_t, int32_t>>::const_iterator>;
+  auto run_of = [](const std::vector<std::pair<int32_t, int32_t>>& list, int32_t e) -> Run {
+    auto lo = std::lower_bound(list.begin(), list.end(), std::make_pair(e, INT32_MIN));
+    auto hi = lo;
+    while (hi != list.end() && hi->first == e) ++hi;
+    return {lo, hi};
+  };
+  // a node's sensor slot: the last sensor entry that names it
+  std::vector<std::pair<int32_t, int32_t>> sens;
+  for (size_t s = 0; s < h->sensors.size(); s++) sens.push_back({h->sensors[s], (int32_t)s});
+  std::sort(sens.begin(), sens.end());
+  // special nodes: a vertex of a cell some force channel covers, or a sensor (internal ids,
+  // ascending)
+  std::vector<int32_t> sp_nodes;
+  for (size_t a = 0; a < cell_k.size(); a++)
+    if (a == 0 || cell_k[a].first != cell_k[a - 1].first)
+      for (int v = 0; v < 8; v++) sp_nodes.push_back(h->node_at[h->cells[(size_t)cell_k[a].first * 8 + v]]);
+  for (int32_t s : h->sensors) sp_nodes.push_back(h->node_at[s]);
+  std::sort(sp_nodes.begin(), sp_nodes.end());
+  sp_nodes.erase(std::unique(sp_nodes.begin(), sp_nodes.end()), sp_nodes.end());
 
   std::vector<double> sp_idt;
   std::vector<int32_t> sp_off, sp_sens, terms;
   const std::vector<double>& idt = h->idt_h;
   std::vector<std::pair<int32_t, int32_t>> node_boxes, cell_boxes;
-  for (int32_t i = 0; i < nn; i++) {
+  for (int32_t i : sp_nodes) {
     const int32_t n = h->node_of[i];
-    if (!is_sp[n]) continue;
     const int32_t sp = (int32_t)sp_idt.size();
     if (h->precision == RSG_FP32 && sp > rsg::kMaxSpecialF32)
       return fail(RSG_ERR_ARG, "too many source/sensor nodes for the FP32 form");
     sp_idt.push_back(idt[i]);
-    sp_sens.push_back(sens_slot[n]);
+    const Run sr = run_of(sens, n);
+    sp_sens.push_back(sr.first == sr.second ? -1 : (sr.second - 1)->second);
     for (int d = 0; d < 3; d++) {
       sp_off.push_back((int32_t)terms.size());
-      if (!any_force) continue;
       for (int kb = 0; kb < 8; kb++) {  // evalXds: slot, then transducer
         const int32_t c = h->node_cells[(size_t)n * 8 + kb];
         if (c == -1) continue;
-        for (int32_t k : cell_ch[c])
-          if (h->channels[k].comp == d) terms.push_back(k);
+        const Run r = run_of(cell_k, c);
+        for (auto it = r.first; it != r.second; ++it)
+          if (h->channels[it->second].comp == d) terms.push_back(it->second);
       }
     }
     sp_off.push_back((int32_t)terms.size());
     node_boxes.push_back({i, sp});
   }
-  // special cells: a vertex lies in a moment channel's support
+  // special cells: a vertex lies in a moment channel's support (internal ids, ascending)
+  std::vector<int32_t> sp_cells;
+  if (!node_k.empty()) {
+    if (h->incidence_exact) {  // the cells around each support node
+      for (size_t a = 0; a < node_k.size(); a++)
+        if (a == 0 || node_k[a].first != node_k[a - 1].first)
+          for (int kb = 0; kb < 8; kb++) {
+            const int32_t c = h->node_cells[(size_t)node_k[a].first * 8 + kb];
+            if (c >= 0) sp_cells.push_back(h->cell_at[c]);
+          }
+      std::sort(sp_cells.begin(), sp_cells.end());
+      sp_cells.erase(std::unique(sp_cells.begin(), sp_cells.end()), sp_cells.end());
+    } else {  // a node slot two cells claim: every cell's vertices
+      std::vector<char> in_support(nn, 0);
+      for (auto& e : node_k) in_support[e.first] = 1;
+      for (int32_t i = 0; i < h->nc; i++) {
+        const int32_t c = h->cell_of[i];
+        bool touched = false;
+        for (int v = 0; v < 8 && !touched; v++) touched = in_support[h->cells[(size_t)c * 8 + v]] != 0;
+        if (touched) sp_cells.push_back(i);
+      }
+    }
+  }
   std::vector<double> sp_dcx;
   std::vector<int32_t> csp_off, cterms;
   const std::vector<double>& dcx = h->dcx_h;
-  if (any_moment) {
-    for (int32_t i = 0; i < nc; i++) {
-      const int32_t c = h->cell_of[i];
-      bool touched = false;
-      for (int v = 0; v < 8 && !touched; v++) touched = !node_ch[h->cells[(size_t)c * 8 + v]].empty();
-      if (!touched) continue;
-      const int32_t sp = (int32_t)sp_dcx.size();
-      if (h->precision == RSG_FP32 && sp > rsg::kMaxSpecialF32)
-        return fail(RSG_ERR_ARG, "too many source cells for the FP32 form");
-      sp_dcx.push_back(dcx[i]);
-      for (int ks = 0; ks < 6; ks++) {
-        csp_off.push_back((int32_t)cterms.size());
-        for (int kb = 0; kb < 8; kb++) {  // evalMoments: vertex, then excitation
-          const int32_t m = h->cells[(size_t)c * 8 + kb];
-          for (int32_t k : node_ch[m])
-            if (h->channels[k].comp == ks) cterms.push_back(k);
-        }
-      }
+  for (int32_t i : sp_cells) {
+    const int32_t c = h->cell_of[i];
+    const int32_t sp = (int32_t)sp_dcx.size();
+    if (h->precision == RSG_FP32 && sp > rsg::kMaxSpecialF32)
+      return fail(RSG_ERR_ARG, "too many source cells for the FP32 form");
+    sp_dcx.push_back(dcx[i]);
+    for (int ks = 0; ks < 6; ks++) {
       csp_off.push_back((int32_t)cterms.size());
-      cell_boxes.push_back({i, sp});
+      for (int kb = 0; kb < 8; kb++) {  // evalMoments: vertex, then excitation
+        const Run r = run_of(node_k, h->cells[(size_t)c * 8 + kb]);
+        for (auto it = r.first; it != r.second; ++it)
+          if (h->channels[it->second].comp == ks) cterms.push_back(it->second);
+      }
     }
+    csp_off.push_back((int32_t)cterms.size());
+    cell_boxes.push_back({i, sp});
   }
   h->n_sp_node = (int32_t)sp_idt.size();
   h->n_sp_cell = (int32_t)sp_dcx.size();
   // class-table form: keys carry the special flag; the special tables carry the class
   std::vector<int32_t> sp_nkey, sp_ckey;
-  std::vector<uint32_t> nkeys, ckeys;
-  if (h->n_nmat >= 0) {
-    nkeys = h->nkey_h;
-    for (auto& b : node_boxes) {
-      sp_nkey.push_back((int32_t)h->nkey_h[b.first]);
-      nkeys[b.first] = rsg::kSpecialKey | (uint32_t)b.second;
-    }
-  }
-  if (h->n_cmat >= 0) {
-    ckeys = h->ckey_h;
-    for (auto& b : cell_boxes) {
-      sp_ckey.push_back((int32_t)h->ckey_h[b.first]);
-      ckeys[b.first] = rsg::kSpecialKey | (uint32_t)b.second;
-    }
-  }
+  if (h->n_nmat >= 0)
+    for (auto& b : node_boxes) sp_nkey.push_back((int32_t)h->nkey_h[b.first]);
+  if (h->n_cmat >= 0)
+    for (auto& b : cell_boxes) sp_ckey.push_back((int32_t)h->ckey_h[b.first]);
 
   auto put_real = [&](DevBuf<char>& buf, const std::vector<double>& v) -> int {
-    int rc = buf.alloc(std::max<size_t>(1, v.size()) * h->elt());
+    int rc = buf.alloc(std::max<size_t>(1, v.size()) * h->elt(), h->stream);
     if (rc) return rc;
     if (v.empty()) return RSG_OK;
-    if (h->precision == RSG_FP64) {
-      HIP_TRY(hipMemcpy(buf.p, v.data(), v.size() * 8, hipMemcpyHostToDevice));
-    } else {
-      std::vector<float> f(v.begin(), v.end());
-      HIP_TRY(hipMemcpy(buf.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
-    }
-    return RSG_OK;
+    if (h->precision == RSG_FP64) return queue_upload(h, buf.p, std::vector<double>(v));
+    return queue_upload(h, buf.p, std::vector<float>(v.begin(), v.end()));
   };
   auto put_int = [&](DevBuf<int32_t>& buf, const std::vector<int32_t>& v) -> int {
-    int rc = buf.alloc(std::max<size_t>(1, v.size()));
+    int rc = buf.alloc(std::max<size_t>(1, v.size()), h->stream);
     if (rc) return rc;
-    if (!v.empty()) HIP_TRY(hipMemcpy(buf.p, v.data(), v.size() * 4, hipMemcpyHostToDevice));
-    return RSG_OK;
+    return queue_upload(h, buf.p, std::vector<int32_t>(v));
   };
   int rc;
   if ((rc = put_real(h->sp_node_val, sp_idt))) return rc;
@@ -1011,17 +1196,17 @@ int build_tables(rsg_instance* h) {
   if ((rc = put_int(h->cell_terms, cterms))) return rc;
   if ((rc = put_int(h->sp_node_key, sp_nkey))) return rc;
   if ((rc = put_int(h->sp_cell_key, sp_ckey))) return rc;
-  if (h->v_lkey) {
-    if ((rc = upload_packed(h->v_lkey, h->v_lconn_h, nkeys))) return rc;
-  } else if (h->nkey) {
-    HIP_TRY(hipMemcpy(h->nkey, nkeys.data(), nkeys.size() * 4, hipMemcpyHostToDevice));
+  if (h->v_lkey || h->nkey) {
+    if ((rc = patch_keys(h, h->v_lkey ? h->v_lkey : h->nkey, h->v_lkey != nullptr, h->v_lconn_h, h->nkey_h,
+                         node_boxes, h->v_flagged)))
+      return rc;
   } else if ((rc = copy_boxed(h, h->idt, idt, node_boxes))) {
     return rc;
   }
-  if (h->s_lkey) {
-    if ((rc = upload_packed(h->s_lkey, h->s_lconn_h, ckeys))) return rc;
-  } else if (h->ckey) {
-    HIP_TRY(hipMemcpy(h->ckey, ckeys.data(), ckeys.size() * 4, hipMemcpyHostToDevice));
+  if (h->s_lkey || h->ckey) {
+    if ((rc = patch_keys(h, h->s_lkey ? h->s_lkey : h->ckey, h->s_lkey != nullptr, h->s_lconn_h, h->ckey_h,
+                         cell_boxes, h->s_flagged)))
+      return rc;
   } else if ((rc = copy_boxed(h, h->dc[0], dcx, cell_boxes))) {
     return rc;
   }
@@ -1030,12 +1215,16 @@ int build_tables(rsg_instance* h) {
   for (int b = 0; b < 2; b++) {
     if (h->stage_ev[b]) HIP_TRY(hipEventSynchronize(h->stage_ev[b]));
     h->stage_busy[b] = false;
-    if ((rc = h->amp[b].alloc(nch * kBlockSteps))) return rc;
-    if (h->stage[b]) (void)hipHostFree(h->stage[b]);
-    h->stage[b] = nullptr;
-    HIP_TRY(hipHostMalloc((void**)&h->stage[b], nch * kBlockSteps * sizeof(double), hipHostMallocDefault));
+    if ((rc = h->amp[b].alloc(nch * kBlockSteps, h->stream))) return rc;
+    // the pinned staging comes with the first rsg_step that uploads amplitudes (a batched
+    // handle never does: hipHostFree, like hipFree, waits for every stream of the device)
+    if (h->stage[b] && h->stage_len[b] < nch * kBlockSteps) {
+      (void)hipHostFree(h->stage[b]);
+      h->stage[b] = nullptr;
+    }
     if (!h->stage_ev[b]) HIP_TRY(hipEventCreateWithFlags(&h->stage_ev[b], hipEventDisableTiming));
   }
+  if ((rc = finish_uploads(h))) return rc;
   h->tables_dirty = false;
   return RSG_OK;
 }
@@ -1047,8 +1236,8 @@ int flush_traces(rsg_instance* h) {
   h->trace_host.resize(old + (size_t)h->trace_dev_steps * row);
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->batch_stream) HIP_TRY(hipStreamSynchronize(h->batch_stream));
-  HIP_TRY(hipMemcpy(h->trace_host.data() + old, h->trace.p,
-                    (size_t)h->trace_dev_steps * row * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_on(h->trace_host.data() + old, h->trace.p, (size_t)h->trace_dev_steps * row * sizeof(double),
+                  hipMemcpyDeviceToHost, h->stream));
   h->trace_dev_steps = 0;
   return RSG_OK;
 }
@@ -1338,7 +1527,11 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   h->cells.assign(mesh->cell_nodes, mesh->cell_nodes + (size_t)nc * 8);
   h->node_cells.assign((size_t)nn * 8, -1);
   for (int32_t k = 0; k < nc; k++)
-    for (int v = 0; v < 8; v++) h->node_cells[(size_t)h->cells[(size_t)k * 8 + v] * 8 + (v ^ 6)] = k;
+    for (int v = 0; v < 8; v++) {
+      int32_t& slot = h->node_cells[(size_t)h->cells[(size_t)k * 8 + v] * 8 + (v ^ 6)];
+      if (slot >= 0) h->incidence_exact = false;
+      slot = k;
+    }
 
   const double* X[3] = {mesh->x, mesh->y, mesh->z};
   CREATE_PHASE("links");
@@ -1532,8 +1725,6 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
         if ((rc = upload_real(h.get(), &h->dn[d], dn[d]))) return rc;
     }
     void* p;
-    if ((rc = upload(h.get(), &p, conn))) return rc;
-    h->ncell = (int32_t*)p;
     LdsPlan plan;
     bool packed = false;
     // Gather lists in chunk words (a quarter of the list bytes; measured 2-5 % faster velocity
@@ -1541,11 +1732,11 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     bool chunk_v = true;
     auto plan_fits = [&]() {
       chunk_v = true;
-      bool ok = build_lds_plan(conn, nn, nc, h->block_v * h->cpt_v, plan, true) &&
+      bool ok = build_lds_plan(conn, nn, h->block_v * h->cpt_v, plan, true) &&
                 rsg::velocity_lds_bytes(plan.max_u, h->elt()) <= lds_limit;
       if (!(ok && (h->n_nmat < 0 || plan.max_u <= rsg::kMaxPackedSet))) {
         LdsPlan plain;
-        if (build_lds_plan(conn, nn, nc, h->block_v * h->cpt_v, plain, false) &&
+        if (build_lds_plan(conn, nn, h->block_v * h->cpt_v, plain, false) &&
             rsg::velocity_lds_bytes(plain.max_u, h->elt()) <= lds_limit) {
           plan = std::move(plain);
           chunk_v = false;
@@ -1571,9 +1762,9 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
       }
     }
     if (fits) {
-      if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
+      if ((rc = upload(h.get(), &p, std::move(plan.u_off)))) return rc;
       h->v_uoff = (int32_t*)p;
-      if ((rc = upload(h.get(), &p, plan.ulist))) return rc;
+      if ((rc = upload(h.get(), &p, std::move(plan.ulist)))) return rc;
       h->v_ulist = (int32_t*)p;
       h->v_maxu = plan.max_u;
       h->v_chunk = chunk_v;
@@ -1583,14 +1774,24 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
         if ((rc = h->dalloc((void**)&h->v_lkey, (size_t)nn * 16))) return rc;
         h->v_lconn_h = std::move(plan.lconn);
       } else {
-        if ((rc = upload(h.get(), &p, plan.lconn))) return rc;
+        if ((rc = upload(h.get(), &p, std::move(plan.lconn)))) return rc;
         h->v_lconn = (uint16_t*)p;
       }
     }
     if (h->n_nmat >= 0 && !packed)
       if ((rc = h->dalloc((void**)&h->nkey, (size_t)nn * 4))) return rc;
-    if ((rc = upload(h.get(), &p, w))) return rc;
+    // every node's plain key (the table builds flag the special ones)
+    if (h->v_lkey) {
+      if ((rc = upload_packed(h.get(), h->v_lkey, h->v_lconn_h, h->nkey_h))) return rc;
+    } else if (h->nkey) {
+      if ((rc = queue_upload(h.get(), h->nkey, std::vector<uint32_t>(h->nkey_h)))) return rc;
+    }
+    if ((rc = upload(h.get(), &p, std::move(w)))) return rc;
     h->node_m = (double*)p;
+    if (!fits) {  // a staged kernel reads its links from the plan, not the connectivity
+      if ((rc = upload(h.get(), &p, std::move(conn)))) return rc;
+      h->ncell = (int32_t*)p;
+    }
     return RSG_OK;
   };
   auto cell_part = [&]() -> int {
@@ -1658,18 +1859,16 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
         }
       for (int d = 0; d < 3; d++)
         if ((rc = upload_real(h.get(), &h->dc[d], dc[d]))) return rc;
-      if ((rc = upload_real(h.get(), &h->C, C))) return rc;
+      if ((rc = upload_real(h.get(), &h->C, std::move(C)))) return rc;
     }
     void* p;
-    if ((rc = upload(h.get(), &p, conn))) return rc;
-    h->cnode = (int32_t*)p;
     LdsPlan plan;
     bool packed = false;
-    if (h->lds_s && build_lds_plan(conn, nc, nn, h->block_s * h->cpt_s, plan) &&
+    if (h->lds_s && build_lds_plan(conn, nc, h->block_s * h->cpt_s, plan) &&
         (size_t)3 * plan.max_u * h->elt() <= lds_limit) {
-      if ((rc = upload(h.get(), &p, plan.u_off))) return rc;
+      if ((rc = upload(h.get(), &p, std::move(plan.u_off)))) return rc;
       h->s_uoff = (int32_t*)p;
-      if ((rc = upload(h.get(), &p, plan.ulist))) return rc;
+      if ((rc = upload(h.get(), &p, std::move(plan.ulist)))) return rc;
       h->s_ulist = (int32_t*)p;
       h->s_maxu = plan.max_u;
       packed = h->n_cmat >= 0 && plan.max_u <= rsg::kMaxPackedSet;
@@ -1677,14 +1876,24 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
         if ((rc = h->dalloc((void**)&h->s_lkey, (size_t)nc * 16))) return rc;
         h->s_lconn_h = std::move(plan.lconn);
       } else {
-        if ((rc = upload(h.get(), &p, plan.lconn))) return rc;
+        if ((rc = upload(h.get(), &p, std::move(plan.lconn)))) return rc;
         h->s_lconn = (uint16_t*)p;
       }
     }
     if (h->n_cmat >= 0 && !packed)
       if ((rc = h->dalloc((void**)&h->ckey, (size_t)nc * 4))) return rc;
-    if ((rc = upload(h.get(), &p, w))) return rc;
+    // every cell's plain key (the table builds flag the special ones)
+    if (h->s_lkey) {
+      if ((rc = upload_packed(h.get(), h->s_lkey, h->s_lconn_h, h->ckey_h))) return rc;
+    } else if (h->ckey) {
+      if ((rc = queue_upload(h.get(), h->ckey, std::vector<uint32_t>(h->ckey_h)))) return rc;
+    }
+    if ((rc = upload(h.get(), &p, std::move(w)))) return rc;
     h->cell_vol = (double*)p;
+    if (!(h->s_lconn || h->s_lkey)) {  // a staged kernel reads its links from the plan
+      if ((rc = upload(h.get(), &p, std::move(conn)))) return rc;
+      h->cnode = (int32_t*)p;
+    }
     return RSG_OK;
   };
   {
@@ -1725,8 +1934,8 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
     if ((rc = h->dalloc(&h->S[s], (size_t)nc * h->elt()))) return rc;
     HIP_TRY(hipMemsetAsync(h->S[s], 0, (size_t)nc * h->elt(), h->stream));
   }
-  if ((rc = h->partial.alloc(2 * rsg::kEnergyMaxBlocks + 2))) return rc;
-  HIP_TRY(hipStreamSynchronize(h->stream));  // this handle's work only (others may be running)
+  if ((rc = h->partial.alloc(2 * rsg::kEnergyMaxBlocks + 2, h->stream))) return rc;
+  if ((rc = finish_uploads(h.get()))) return rc;  // this handle's stream only (others may be running)
   CREATE_PHASE("state");
   *out = h.release();
   return RSG_OK;
@@ -1786,7 +1995,7 @@ int rsg_set_sensors(rsg_handle h, int32_t n, const int32_t* nodes) {
   h->sensors.assign(nodes, nodes + n);
   h->trace_host.clear();
   h->trace_dev_steps = 0;
-  int rc = h->trace.alloc((size_t)std::max(1, n) * 3 * kTraceCapSteps);
+  int rc = h->trace.alloc((size_t)std::max(1, n) * 3 * kTraceCapSteps, h->stream);
   if (rc) return rc;
   h->tables_dirty = true;
   return build_tables(h);
@@ -1811,6 +2020,10 @@ int rsg_step(rsg_handle h, int32_t nsteps, const double* amplitudes, double* tim
     const int b = h->cur;
     if (nch) {
       if (h->stage_busy[b]) HIP_TRY(hipEventSynchronize(h->stage_ev[b]));
+      if (!h->stage[b]) {
+        HIP_TRY(hipHostMalloc((void**)&h->stage[b], nch * kBlockSteps * sizeof(double), hipHostMallocDefault));
+        h->stage_len[b] = nch * kBlockSteps;
+      }
       double* st = h->stage[b];
       double t = h->time;
       for (int k = 0; k < kb; k++) {
@@ -1853,7 +2066,7 @@ int rsg_get_state(rsg_handle h, double* V, double* S) {
   const size_t e = h->elt();
   std::vector<char> buf((size_t)std::max(h->nn, h->nc) * e);
   auto fetch = [&](void* src, int32_t n, const std::vector<int32_t>& of, double* dst) -> int {
-    HIP_TRY(hipMemcpy(buf.data(), src, (size_t)n * e, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(buf.data(), src, (size_t)n * e, hipMemcpyDeviceToHost, h->stream));
     if (e == 8) {
       const double* b = (const double*)buf.data();
       for (int32_t i = 0; i < n; i++) dst[of[i]] = b[i];
@@ -1887,7 +2100,7 @@ int rsg_set_state(rsg_handle h, const double* V, const double* S) {
       float* b = (float*)buf.data();
       for (int32_t i = 0; i < n; i++) b[i] = (float)src[of[i]];
     }
-    HIP_TRY(hipMemcpy(dst, buf.data(), (size_t)n * e, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(dst, buf.data(), (size_t)n * e, hipMemcpyHostToDevice, h->stream));
     return RSG_OK;
   };
   int rc;
@@ -2371,15 +2584,16 @@ int batch_block(rsg_batch_s* b, int l, int32_t* unfinished) {
     }
     amp_off += nch * (size_t)b->K;
   }
-  // groups: instances of one launch configuration (coefficient form, workgroup size and
-  // per-thread count of each kernel, so each launch runs one kernel variant), in slot order
-  // within a configuration, each group within the state budget (an instance larger than the
-  // budget forms a group of its own)
-  auto config = [](const rsg_instance* h) -> std::array<int32_t, 6> {
+  // groups: instances of one launch configuration (coefficient form, workgroup size,
+  // per-thread count and LDS staging of each kernel, so each launch runs one kernel variant;
+  // a member with a plan keeps no connectivity on the device), in slot order within a
+  // configuration, each group within the state budget (an instance larger than the budget
+  // forms a group of its own)
+  auto config = [](const rsg_instance* h) -> std::array<int32_t, 8> {
     return {h->v_lkey ? 3 : h->nkey ? 1 : 0, h->s_lkey ? 3 : h->ckey ? 1 : 0, h->block_v, h->block_s, h->cpt_v,
-            h->cpt_s};
+            h->cpt_s, (h->v_lconn || h->v_lkey) ? 1 : 0, (h->s_lconn || h->s_lkey) ? 1 : 0};
   };
-  std::vector<std::array<int32_t, 6>> configs;  // distinct configurations, first seen first
+  std::vector<std::array<int32_t, 8>> configs;  // distinct configurations, first seen first
   for (size_t i = 0; i < b->slots.size(); i++)
     if (kmax[i]) {
       const auto f = config(b->slots[i].h);
@@ -2410,7 +2624,8 @@ int batch_block(rsg_batch_s* b, int l, int32_t* unfinished) {
       if (ov + table_words(nv) > ln.cap_wv || os + table_words(ns) > ln.cap_ws)
         return fail(RSG_ERR_STATE, "batch work tables overflow");
       const int32_t qv = (int32_t)padded_items(nv), qs = (int32_t)padded_items(ns);
-      // LDS staging for a kernel when every member has a plan, sized for the largest set
+      // LDS staging for a kernel when its members have plans (all or none do), sized for the
+      // largest set
       bool lds_v = true, lds_s = true;
       int32_t mu_v = 0, mu_s = 0;
       for (size_t e : members) {

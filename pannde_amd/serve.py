@@ -236,6 +236,7 @@ class Server:
         now = lambda: time.perf_counter() - start  # noqa: E731
         st = dict(nxt=0, used_bytes=0, resident_cells=0, cell_steps=0, blocks=0)
         preps = {}  # future -> (request, bytes, admitted time, is spot check)
+        fetches = {}  # future -> job: a finished solve's results read back on a pool thread
         active: Dict[int, _Job] = {}  # main batch: slot -> job
         checking: Dict[int, _Job] = {}  # FP64 batch: slot -> job
         results: Dict[int, Dict] = {}
@@ -269,10 +270,13 @@ class Server:
                                   latency=t_done - r.arrival * time_scale, status=status, error=error, **extra)
 
         def collect(timeout):
-            if not preps:
+            if not preps and not fetches:
                 return
-            done, _ = cf.wait(list(preps), timeout=timeout, return_when=cf.FIRST_COMPLETED)
+            done, _ = cf.wait(list(preps) + list(fetches), timeout=timeout, return_when=cf.FIRST_COMPLETED)
             for f in done:
+                if f in fetches:
+                    fetched(fetches.pop(f), f.result())
+                    continue
                 r, est, t_adm, is_check = preps.pop(f)
                 try:
                     case, s, secs = f.result()
@@ -325,51 +329,73 @@ class Server:
             st["blocks"] += 1
             return [k for k in jobs if k not in before and batch.status(k)[0] >= batch.status(k)[1]]
 
-        def retire_main(slot):
-            job = active.pop(slot)
-            main.remove(slot)  # waits only for the block in which it finished
-            tr = job.solver.traces()
-            # a diverged solve (non-finite traces or fields) is reported failed, not returned
-            finite = bool(np.isfinite(tr).all()) and all(math.isfinite(e) for e in job.solver.energy())
-            t_done = now()
+        def fetch(job, is_check):
+            """On a pool thread: the finished solve's traces and energies, then its tear-down.
+            The host waits on the solver's own stream here, not in the loop that feeds the GPU.
+            Returns (traces, error or None, time)."""
+            try:
+                tr = job.solver.traces()
+                err = None
+                # a diverged solve (non-finite traces or fields) is reported failed
+                if not is_check and not (np.isfinite(tr).all() and all(math.isfinite(e) for e in job.solver.energy())):
+                    err = "non-finite result (NaN or Inf in traces or fields)"
+            except Exception as e:  # this request only
+                tr, err = None, "%s: %s" % (type(e).__name__, e)
+            finally:
+                job.solver.close()
+            return tr, err, now()
+
+        def fetched(job, out):
+            """A fetched result: recorded, its budget released (its memory is free by now)."""
+            tr, error, t_done = out
+            finite = error is None
             r = job.req
             st["cell_steps"] += job.case.n_cells * r.nsteps
+            st["used_bytes"] -= job.bytes
+            if job.slot < 0:  # an FP64 spot check: compared with the request's FP32 traces
+                res = results[r.rid]
+                if error is not None:
+                    res["spot_check_error"] = error
+                    res.pop("fp32_traces", None)
+                    return
+                err = rel_l2_traces(res.pop("fp32_traces"), tr)
+                # None when no sensor saw the wave in the FP64 run (nothing to compare)
+                res["spot_check_trace_rel_l2"] = float(err.max()) if err.size else None
+                res["spot_check_sensors"] = int(err.size)
+                if err.size:
+                    spot_errors.append(res["spot_check_trace_rel_l2"])
+                return
             st["resident_cells"] -= r.cells
             results[r.rid] = dict(rid=r.rid, kind=r.kind, cells=job.case.n_cells, nsteps=r.nsteps,
                                   arrival=r.arrival * time_scale, admitted=job.admitted, ready=job.ready,
                                   finished=t_done, latency=t_done - r.arrival * time_scale,
-                                  status="ok" if finite else "failed",
-                                  error=None if finite else "non-finite result (NaN or Inf in traces or fields)")
+                                  status="ok" if finite else "failed", error=error)
             if not finite:
-                self.log("serve: request %d diverged; reported failed" % r.rid)
+                self.log("serve: request %d failed: %s" % (r.rid, error))
             if self.keep_traces:
                 results[r.rid]["traces"] = tr
-            pool.submit(job.solver.close)
-            if r.spot_check and finite:  # the FP64 re-run shares the byte budget; its solve reuses the case
+            if r.spot_check and finite:  # the FP64 re-run reuses the case
                 est = estimate_device_bytes(job.case.n_cells, job.case.n_nodes, rsg.RSG_FP64)
-                st["used_bytes"] += est - job.bytes
+                st["used_bytes"] += est
                 preps[pool.submit(self._prepare, r, rsg.RSG_FP64, job.case)] = (r, est, t_done, True)
                 results[r.rid]["fp32_traces"] = tr
-            else:
-                st["used_bytes"] -= job.bytes
+
+        def retire_main(slot):
+            job = active.pop(slot)
+            main.remove(slot)  # its last block ran one block ago: usually no wait
+            fetches[pool.submit(fetch, job, False)] = job
 
         def retire_check(slot):
             job = checking.pop(slot)
             check.remove(slot)
-            tr64 = job.solver.traces()
-            res = results[job.req.rid]
-            err = rel_l2_traces(res.pop("fp32_traces"), tr64)
-            # None when no sensor saw the wave in the FP64 run (nothing to compare)
-            res["spot_check_trace_rel_l2"] = float(err.max()) if err.size else None
-            res["spot_check_sensors"] = int(err.size)
-            if err.size:
-                spot_errors.append(res["spot_check_trace_rel_l2"])
-            st["cell_steps"] += job.case.n_cells * job.req.nsteps
-            st["used_bytes"] -= job.bytes
-            pool.submit(job.solver.close)
+            job.slot = -1
+            fetches[pool.submit(fetch, job, True)] = job
 
-        pending_main, pending_check = [], []
-        while st["nxt"] < len(reqs) or preps or active or checking:
+        # the slots that finished in each of the last two enqueued blocks, older first: a solve is
+        # retired one block after the block it finished in, so that its removal finds that
+        # block done while the next one still runs
+        pending_main, pending_check = [[], []], [[], []]
+        while st["nxt"] < len(reqs) or preps or active or checking or fetches:
             if now() - last_log >= 15.0:
                 last_log = now()
                 self.log("serve: t=%.0f s, %d done, %d active, %d preparing, %d checking, %d waiting"
@@ -383,19 +409,20 @@ class Server:
             if self.on_block and st["blocks"] > blocks0:
                 self.on_block([j.req.rid for j in active.values()])
             new_check = enqueue(check, checking)
-            for slot in pending_main:
+            for slot in pending_main.pop(0):
                 retire_main(slot)
-            for slot in pending_check:
+            for slot in pending_check.pop(0):
                 retire_check(slot)
-            pending_main, pending_check = new_main, new_check
-            if not new_main and not new_check and not pending_main and not pending_check:
+            pending_main.append(new_main)
+            pending_check.append(new_check)
+            if not new_main and not new_check and not any(pending_main) and not any(pending_check):
                 if not any(main.status(k)[0] < main.status(k)[1] for k in active) and \
                         not any(check.status(k)[0] < check.status(k)[1] for k in checking):
                     # nothing on the GPU: wait for a solver to be ready or the next arrival
                     wait = None
                     if st["nxt"] < len(reqs):
                         wait = max(0.0, reqs[st["nxt"]].arrival * time_scale - now())
-                    if preps:
+                    if preps or fetches:
                         collect(wait if wait is not None else 1.0)
                     elif wait is not None:
                         time.sleep(min(wait, 0.05))

@@ -84,6 +84,15 @@ hipError_t hipFree(void* p) {
   }
   return hipSuccess;
 }
+// the stream-ordered pool: its allocations are host memory like hipMalloc's
+hipError_t hipMemPoolCreate(hipMemPool_t* pool, const hipMemPoolProps*) {
+  static int token;
+  *pool = reinterpret_cast<hipMemPool_t>(&token);
+  return hipSuccess;
+}
+hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void*) { return hipSuccess; }
+hipError_t hipMallocFromPoolAsync(void** p, size_t bytes, hipMemPool_t, hipStream_t) { return hipMalloc(p, bytes); }
+hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
 hipError_t hipHostMalloc(void** p, size_t bytes, unsigned int) { return hipMalloc(p, bytes); }
 hipError_t hipHostFree(void* p) { return hipFree(p); }
 hipError_t hipMemcpy(void* dst, const void* src, size_t n, hipMemcpyKind) {
@@ -169,35 +178,49 @@ namespace rsg {
 
 // The launchers check what the real ones check and touch every table entry a kernel would
 // read first (so an out-of-range table shows up under the sanitizer); nothing is computed.
+// links: a staged launch reads the block-local links, an unstaged one the connectivity
 template <typename A>
-static hipError_t check_block(const A& a, int block) {
+static bool has_links(const A& a, const int32_t* conn, bool lds) {
+  return lds ? (a.lconn != nullptr || a.lkey != nullptr) : conn != nullptr;
+}
+template <typename A>
+static hipError_t check_block(const A& a, int block, const int32_t* conn) {
   if (block < 64 || block > 512 || block % 64 || a.cpt < 1) return hipErrorInvalidValue;
+  if (!has_links(a, conn, a.lconn != nullptr || a.lkey != nullptr)) return hipErrorInvalidValue;
   return hipSuccess;
 }
-hipError_t launch_velocity_f64(const VelocityArgs<double>& a, int block, hipStream_t) { return check_block(a, block); }
-hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStream_t) { return check_block(a, block); }
-hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t) { return check_block(a, block); }
-hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t) { return check_block(a, block); }
+hipError_t launch_velocity_f64(const VelocityArgs<double>& a, int block, hipStream_t) {
+  return check_block(a, block, a.ncell);
+}
+hipError_t launch_velocity_f32(const VelocityArgs<float>& a, int block, hipStream_t) {
+  return check_block(a, block, a.ncell);
+}
+hipError_t launch_stress_f64(const StressArgs<double>& a, int block, hipStream_t) { return check_block(a, block, a.cnode); }
+hipError_t launch_stress_f32(const StressArgs<float>& a, int block, hipStream_t) { return check_block(a, block, a.cnode); }
 
 template <typename T>
-static hipError_t batch(const BatchDesc<T>* d, const int32_t* table, int n, int block, hipGraph_t graph,
-                        hipGraphNode_t* node) {
+static hipError_t batch(const BatchDesc<T>* d, const int32_t* table, int n, int block, bool lds, bool velocity,
+                        hipGraph_t graph, hipGraphNode_t* node) {
   if (block < 64 || block > 512 || block % 64) return hipErrorInvalidValue;
   volatile long sink = 0;
   for (int i = 0; i < n; i++) {
     const int32_t w = table[2 * i];
-    if (w >= 0) sink = sink + d[w & (kMaxBatchSlots - 1)].kmax;
+    if (w < 0) continue;
+    const BatchDesc<T>& m = d[w & (kMaxBatchSlots - 1)];
+    sink = sink + m.kmax;
+    // every member of a launch has what the launch's form reads
+    if (velocity ? !has_links(m.v, m.v.ncell, lds) : !has_links(m.s, m.s.cnode, lds)) return hipErrorInvalidValue;
   }
   if (graph && node) *node = reinterpret_cast<hipGraphNode_t>(graph);
   return hipSuccess;
 }
-hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* t, int n, int, int, int block, bool, int, int,
-                            int, bool, hipStream_t, hipGraph_t g, hipGraphNode_t* node) {
-  return batch(d, t, n, block, g, node);
+hipError_t launch_batch_f64(const BatchDesc<double>* d, const int32_t* t, int n, int, int, int block, bool lds, int,
+                            int, int, bool velocity, hipStream_t, hipGraph_t g, hipGraphNode_t* node) {
+  return batch(d, t, n, block, lds, velocity, g, node);
 }
-hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* t, int n, int, int, int block, bool, int, int,
-                            int, bool, hipStream_t, hipGraph_t g, hipGraphNode_t* node) {
-  return batch(d, t, n, block, g, node);
+hipError_t launch_batch_f32(const BatchDesc<float>* d, const int32_t* t, int n, int, int, int block, bool lds, int,
+                            int, int, bool velocity, hipStream_t, hipGraph_t g, hipGraphNode_t* node) {
+  return batch(d, t, n, block, lds, velocity, g, node);
 }
 hipError_t energy_scratch(int device, double** out) {
   static double area[2 * kEnergyMaxBlocks + 2];

@@ -208,19 +208,24 @@ void batches(const Box& b, const Box& small) {
       EXPECT(rsg_batch_set_option(bt, RSG_BATCH_GROUP_BYTES, 1 << 20) == RSG_OK);
       rsg_tuning t{};
       t.per_thread_velocity = 2;
-      rsg_handle h[3] = {make_solver(b, precision, nullptr, 0, 4), make_solver(small, precision, &t, 1, 3),
-                         make_solver(small, precision, nullptr, 0, 0)};
-      int32_t slot[3];
-      const int64_t steps[3] = {70, 33, 100};
+      // the last two differ only in LDS staging (a staged member keeps no connectivity on
+      // the device, so the two must not share a launch)
+      rsg_tuning unstaged{};
+      unstaged.lds_velocity = -1;
+      unstaged.lds_stress = -1;
+      rsg_handle h[4] = {make_solver(b, precision, nullptr, 0, 4), make_solver(small, precision, &t, 1, 3),
+                         make_solver(small, precision, nullptr, 0, 0), make_solver(small, precision, &unstaged, 0, 2)};
+      int32_t slot[4];
+      const int64_t steps[4] = {70, 33, 100, 40};
       for (int i = 0; i < 2; i++) EXPECT(rsg_batch_add(bt, h[i], steps[i], &slot[i]) == RSG_OK);
       EXPECT(rsg_step(h[0], 1, nullptr, nullptr) == RSG_ERR_STATE);
       int32_t left = -1;
       EXPECT(rsg_batch_run(bt, 2, &left) == RSG_OK);
-      EXPECT(rsg_batch_add(bt, h[2], steps[2], &slot[2]) == RSG_OK);  // joins between blocks
+      for (int i = 2; i < 4; i++) EXPECT(rsg_batch_add(bt, h[i], steps[i], &slot[i]) == RSG_OK);  // join later
       while (left > 0) EXPECT(rsg_batch_run(bt, 1, &left) == RSG_OK);
       EXPECT(rsg_batch_run(bt, 3, &left) == RSG_OK && left == 0);
       EXPECT(rsg_batch_synchronize(bt) == RSG_OK);
-      for (int i = 0; i < 3; i++) {
+      for (int i = 0; i < 4; i++) {
         int64_t done = 0, total = 0;
         EXPECT(rsg_batch_slot_status(bt, slot[i], &done, &total) == RSG_OK && done == steps[i] && total == steps[i]);
         EXPECT(rsg_batch_remove(bt, slot[i]) == RSG_OK);
@@ -229,6 +234,27 @@ void batches(const Box& b, const Box& small) {
       for (auto x : h)
         if (x) EXPECT(rsg_destroy(x) == RSG_OK);
     }
+  // one lane, one unbounded group budget: a staged and an unstaged member of the same
+  // coefficient form (per-entity arrays) and workgroup shape still take separate launches
+  for (int precision : {RSG_FP64, RSG_FP32}) {
+    rsg_batch bt = nullptr;
+    EXPECT(rsg_batch_create(precision, 8, 0, 1, &bt) == RSG_OK);
+    EXPECT(rsg_batch_set_option(bt, RSG_BATCH_LANES, 1) == RSG_OK);
+    EXPECT(rsg_batch_set_option(bt, RSG_BATCH_GROUP_BYTES, 0) == RSG_OK);
+    rsg_tuning staged{}, unstaged{};
+    staged.coeff_classes = unstaged.coeff_classes = -1;
+    unstaged.lds_velocity = -1;
+    unstaged.lds_stress = -1;
+    rsg_handle h[2] = {make_solver(small, precision, &staged, 0, 1), make_solver(small, precision, &unstaged, 0, 1)};
+    int32_t slot[2];
+    for (int i = 0; i < 2; i++) EXPECT(rsg_batch_add(bt, h[i], 20, &slot[i]) == RSG_OK);
+    int32_t left = -1;
+    for (int q = 0; q < 8 && left != 0; q++) EXPECT(rsg_batch_run(bt, 1, &left) == RSG_OK);
+    EXPECT(left == 0);
+    for (int i = 0; i < 2; i++) EXPECT(rsg_batch_remove(bt, slot[i]) == RSG_OK);
+    EXPECT(rsg_batch_destroy(bt) == RSG_OK);
+    for (auto x : h) EXPECT(rsg_destroy(x) == RSG_OK);
+  }
 }
 
 }  // namespace
