@@ -223,7 +223,9 @@ int rsg_get_kernel_times(rsg_handle h, double* velocity_ms, double* stress_ms,
  * and trace buffers). */
 int rsg_device_bytes(rsg_handle h, int64_t* bytes);
 
-/* Releases the handle and its device memory. */
+/* Releases the handle and returns its device memory to the library's pool (one per device,
+ * kept for later handles until the process ends) in order behind the handle's own stream,
+ * without waiting for other streams. */
 int rsg_destroy(rsg_handle h);
 
 /* Copies the calling thread's last error message into buf (NUL-terminated); returns its
