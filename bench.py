@@ -43,9 +43,10 @@ HBM_PEAK_GBS = 8000.0
 #   FP32: stress 188, velocity 96.  This layout moves fewer bytes than that model, so the model is
 #   reported separately and the roofline uses the layout's own algorithmic bytes.
 SURVEY_MODEL_BYTES = {"stress": {"f64": 344, "f32": 188}, "velocity": {"f64": 160, "f32": 96}}
-# Every 8th step's two launches are bracketed by HIP events inside the timed region: events
-# between every launch slowed the stream by 4 % (FP64) to 8 % (FP32).
-TIMING_STRIDE = 8
+# Every 32nd step's two launches are bracketed by HIP events inside the timed region (about 60
+# samples per kernel in a 2000-step run): events between every launch slowed the stream by 4 %
+# (FP64) to 8 % (FP32).
+TIMING_STRIDE = 32
 # the sources the counter traffic in profiles/ was measured with
 KERNEL_SOURCES = ("pannde_amd/csrc/rsg_kernels.hip", "pannde_amd/csrc/rsg_internal.h")
 # Untimed first steps of the drop-in driver leg (code-object loading on first launch).
