@@ -62,6 +62,9 @@ def build(verbose=False):
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs +
          ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp], verbose)
     os.replace(tmp, OUT)
+    # the native mesh builder of the case generator (host code only)
+    _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-shared", "-fPIC",
+          os.path.join(CSRC, "case_gen.cpp"), "-o", os.path.join(HERE, "libpannde_cases.so")], verbose)
     # the C++ drop-in driver (PanNDE::Model over the C ABI; VTK XML jobs through vtk_job.hpp,
     # zlib for compressed arrays), linked against the library
     inc = os.path.join(ROOT, "include")

@@ -22,8 +22,10 @@ Nothing here touches a GPU; the arrays feed ``pannde_amd.rsg.DeviceSolver``.
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import math
+import os
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -83,12 +85,57 @@ class Case:
 
 # ---------------------------------------------------------------- meshes
 
-def structured_mesh(nx, ny, nz, dx, dy, dz, keep=None, origin=(0.0, 0.0, 0.0)):
+_NATIVE = None  # libpannde_cases.so (the native mesh builder), False when it is not built
+
+
+def _native():
+    global _NATIVE
+    if _NATIVE is None:
+        try:
+            lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpannde_cases.so"))
+        except OSError:
+            _NATIVE = False
+            return None
+        i32, p = ctypes.c_int32, ctypes.c_void_p
+        lib.cg_mesh_count.argtypes = [i32, i32, i32, p, p, p]
+        lib.cg_mesh_fill.argtypes = [i32, i32, i32, p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     p, p, p, p, p, p, p]
+        _NATIVE = lib
+    return _NATIVE or None
+
+
+def structured_mesh(nx, ny, nz, dx, dy, dz, keep=None, origin=(0.0, 0.0, 0.0), cell_grid=False):
     """Right-hex grid of nx*ny*nz cells in case-maker numbering.
 
     ``keep`` (bool [nx, ny, nz]) removes cells; nodes that no kept cell uses are dropped
     and the rest renumbered in the same loop order.  Coordinates are ``origin + k*d``.
-    Returns (x, y, z, cells, cell_ijk)."""
+    Returns (x, y, z, cells, cell_ijk), and with ``cell_grid`` also the [nx, ny, nz] int32
+    grid of cell numbers (-1 where removed).  The arrays are built by the native builder
+    (pannde_amd/csrc/case_gen.cpp) when it is built, else by structured_mesh_numpy; the two
+    are equal array for array."""
+    if keep is None:
+        keep = np.ones((nx, ny, nz), dtype=bool)
+    keep = np.ascontiguousarray(keep, dtype=bool)
+    lib = _native()
+    if lib is None:
+        return structured_mesh_numpy(nx, ny, nz, dx, dy, dz, keep, origin, cell_grid)
+    k8 = keep.view(np.uint8)
+    nn, nc = ctypes.c_int64(0), ctypes.c_int64(0)
+    if lib.cg_mesh_count(nx, ny, nz, k8.ctypes.data, ctypes.addressof(nn), ctypes.addressof(nc)) != 0:
+        raise ValueError("grid %dx%dx%d is too large" % (nx, ny, nz))
+    x, y, z = (np.empty(nn.value) for _ in range(3))
+    cells = np.empty((nc.value, 8), dtype=np.int32)
+    ijk = np.empty((nc.value, 3), dtype=np.int64)
+    grid = np.empty((nx, ny, nz), dtype=np.int32) if cell_grid else None
+    org = np.asarray(origin, dtype=np.float64)
+    lib.cg_mesh_fill(nx, ny, nz, k8.ctypes.data, float(dx), float(dy), float(dz), org.ctypes.data, x.ctypes.data,
+                     y.ctypes.data, z.ctypes.data, cells.ctypes.data, ijk.ctypes.data,
+                     grid.ctypes.data if cell_grid else None)
+    return (x, y, z, cells, ijk, grid) if cell_grid else (x, y, z, cells, ijk)
+
+
+def structured_mesh_numpy(nx, ny, nz, dx, dy, dz, keep=None, origin=(0.0, 0.0, 0.0), cell_grid=False):
+    """structured_mesh in numpy: the reference the native builder is tested against."""
     if keep is None:
         keep = np.ones((nx, ny, nz), dtype=bool)
     keep = np.asarray(keep, dtype=bool)
@@ -109,8 +156,13 @@ def structured_mesh(nx, ny, nz, dx, dy, dz, keep=None, origin=(0.0, 0.0, 0.0)):
     cells = np.empty((ci.shape[0], 8), dtype=np.int32)
     for v, (a, b, c) in enumerate(offs):
         cells[:, v] = nid[a:a + nx, b:b + ny, c:c + nz][keep]
-    return (np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64), np.asarray(z, dtype=np.float64),
-            cells, np.stack([ci, cj, ck], axis=1))
+    out = (np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64), np.asarray(z, dtype=np.float64),
+           cells, np.stack([ci, cj, ck], axis=1))
+    if not cell_grid:
+        return out
+    grid = np.full(keep.shape, -1, dtype=np.int32)
+    grid[keep] = np.arange(ci.shape[0], dtype=np.int32)
+    return out + (grid,)
 
 
 def renumber_nodes(x, y, z, cells, new_of_old):
@@ -283,10 +335,10 @@ def cfrp_plate(seed=2, nx=256, ny=256, cells_per_ply=4, dx=5.0e-5, nsteps=2000, 
     for hx, hy, r in holes:
         keep2d &= (xc[:, None] - hx) ** 2 + (yc[None, :] - hy) ** 2 >= r * r
     keep = np.repeat(keep2d[:, :, None], nz, axis=2)
-    x, y, z, cells, ijk = structured_mesh(nx, ny, nz, dx, dx, dx, keep=keep)
+    x, y, z, cells, ijk, grid = structured_mesh(nx, ny, nz, dx, dx, dx, keep=keep, cell_grid=True)
     n_dup = 0
     if crack is not None:
-        x, y, z, cells, n_dup = surface_crack(x, y, z, cells, ijk, (dx, dx, dx), crack)
+        x, y, z, cells, n_dup = surface_crack(x, y, z, cells, ijk, (dx, dx, dx), crack, cell_grid=grid)
     ply_c = np.stack([rotate_c_about_z(cfrp_ply_c(), a) for a in CFRP_LAYUP])
     ply_of_cell = ijk[:, 2] // cells_per_ply
     rho = 1580.0
@@ -316,7 +368,7 @@ class Crack:
     depth: float
 
 
-def surface_crack(x, y, z, cells, ijk, d, crack: Crack, origin=(0.0, 0.0, 0.0)):
+def surface_crack(x, y, z, cells, ijk, d, crack: Crack, origin=(0.0, 0.0, 0.0), cell_grid=None):
     """Opens ``crack`` in a right-hex mesh on a grid of spacing d = (dx, dy, dz) by
     duplicating nodes, as a mesh file describes a crack (coincident nodes on the two
     faces, each face's cells using their own copy).
@@ -325,7 +377,10 @@ def surface_crack(x, y, z, cells, ijk, d, crack: Crack, origin=(0.0, 0.0, 0.0)):
     cells of opposite sides is cracked when its centre is within the crack rectangle.  A
     node is duplicated when every opposite-side face around it is cracked (so the crack
     tip stays bonded and the mouth at the surface opens); the negative side's cells take
-    the copy, appended after the existing nodes.  Returns (x, y, z, cells, n_dup)."""
+    the copy, appended after the existing nodes.  Returns (x, y, z, cells, n_dup).
+
+    ``cell_grid`` (structured_mesh's grid of cell numbers) finds the region's cells without a
+    pass over the whole mesh; the result is the same."""
     dx, dy, dz = d
     th = math.radians(crack.angle_deg)
     nrm = np.array([math.cos(th), math.sin(th), 0.0])
@@ -338,16 +393,29 @@ def surface_crack(x, y, z, cells, ijk, d, crack: Crack, origin=(0.0, 0.0, 0.0)):
     hi = np.array([cx + ext[0] + 2 * dx, cy + ext[1] + 2 * dy, ztop + 2 * dz])
     g_lo = np.floor((lo - np.asarray(origin)) / np.array(d)).astype(np.int64)
     g_hi = np.ceil((hi - np.asarray(origin)) / np.array(d)).astype(np.int64)
-    g_lo = np.maximum(g_lo, ijk.min(axis=0))
-    g_hi = np.minimum(g_hi, ijk.max(axis=0) + 1)
-    if np.any(g_hi <= g_lo):
-        return x, y, z, cells, 0
-    shape = tuple(int(v) for v in g_hi - g_lo)
-    inreg = np.all((ijk >= g_lo) & (ijk < g_hi), axis=1)
-    reg_cells = np.nonzero(inreg)[0]
-    loc = ijk[reg_cells] - g_lo
-    cgrid = np.full(shape, -1, dtype=np.int64)
-    cgrid[loc[:, 0], loc[:, 1], loc[:, 2]] = reg_cells
+    if cell_grid is not None:
+        # the region clipped to the grid rather than to the kept cells: the extra rows hold
+        # no cells, so the faces, and the duplicated nodes in C order, are the same
+        g_lo = np.maximum(g_lo, 0)
+        g_hi = np.minimum(g_hi, np.array(cell_grid.shape))
+        if np.any(g_hi <= g_lo):
+            return x, y, z, cells, 0
+        shape = tuple(int(v) for v in g_hi - g_lo)
+        cgrid = cell_grid[g_lo[0]:g_hi[0], g_lo[1]:g_hi[1], g_lo[2]:g_hi[2]].astype(np.int64)
+        li, lj, lk = np.nonzero(cgrid >= 0)  # C order: ascending cell numbers
+        loc = np.stack([li, lj, lk], axis=1)
+        reg_cells = cgrid[li, lj, lk]
+    else:
+        g_lo = np.maximum(g_lo, ijk.min(axis=0))
+        g_hi = np.minimum(g_hi, ijk.max(axis=0) + 1)
+        if np.any(g_hi <= g_lo):
+            return x, y, z, cells, 0
+        shape = tuple(int(v) for v in g_hi - g_lo)
+        inreg = np.all((ijk >= g_lo) & (ijk < g_hi), axis=1)
+        reg_cells = np.nonzero(inreg)[0]
+        loc = ijk[reg_cells] - g_lo
+        cgrid = np.full(shape, -1, dtype=np.int64)
+        cgrid[loc[:, 0], loc[:, 1], loc[:, 2]] = reg_cells
     # side of each region cell (+1 / -1) from its centre
     ii, jj, kk = np.meshgrid(*(np.arange(n) for n in shape), indexing="ij")
     gx = origin[0] + (g_lo[0] + ii + 0.5) * dx
@@ -418,7 +486,7 @@ def ti_bracket(seed=5, leg=512, width=96, height=96, dx=5.0e-5, nsteps=4000, cra
     cj = np.arange(ny)[None, :]
     keep2d = (cj < width) | (ci < width)
     keep = np.repeat(keep2d[:, :, None], height, axis=2)
-    x, y, z, cells, ijk = structured_mesh(nx, ny, height, dx, dx, dx, keep=keep)
+    x, y, z, cells, ijk, grid = structured_mesh(nx, ny, height, dx, dx, dx, keep=keep, cell_grid=True)
     L = leg * dx
     W = width * dx
     H = height * dx
@@ -429,7 +497,7 @@ def ti_bracket(seed=5, leg=512, width=96, height=96, dx=5.0e-5, nsteps=4000, cra
     ccx = rng.uniform(0.55 * L, 0.8 * L)
     ccy = rng.uniform(0.3 * W, 0.7 * W)
     crack = Crack((ccx, ccy, H), angle, length, depth)
-    x, y, z, cells, n_dup = surface_crack(x, y, z, cells, ijk, (dx, dx, dx), crack)
+    x, y, z, cells, n_dup = surface_crack(x, y, z, cells, ijk, (dx, dx, dx), crack, cell_grid=grid)
     mat = TI64
     c6 = isotropic_c(mat["lam"], mat["mu"])
     nc = cells.shape[0]

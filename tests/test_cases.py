@@ -157,3 +157,58 @@ def test_generators_are_stable(name):
         "served_request": lambda: serve.build_case(serve.request_stream(seed=9, n=3, rate=4.0)[1]),
     }[name]
     assert _digest(make()) == GENERATOR_DIGESTS[name]
+
+
+def _mesh_keeps(rng):
+    yield (1, 1, 1), np.ones((1, 1, 1), dtype=bool)
+    yield (5, 4, 3), np.ones((5, 4, 3), dtype=bool)
+    yield (6, 5, 4), rng.random((6, 5, 4)) < 0.7  # scattered removals, orphan nodes
+    yield (4, 4, 2), np.zeros((4, 4, 2), dtype=bool)  # nothing kept
+    holed = np.ones((12, 10, 3), dtype=bool)
+    holed[3:7, 2:5, :] = False  # a through-hole
+    yield (12, 10, 3), holed
+
+
+@pytest.mark.skipif(cases._native() is None, reason="libpannde_cases.so is not built")
+def test_native_mesh_builder_equals_numpy():
+    """pannde_amd/csrc/case_gen.cpp against structured_mesh_numpy: every array (coordinates,
+    vertex lists, cell indices, cell grid) equal in value and dtype."""
+    rng = np.random.default_rng(17)
+    for (nx, ny, nz), keep in _mesh_keeps(rng):
+        for origin in ((0.0, 0.0, 0.0), (-1.25e-3, 3.0e-4, 7.5e-5)):
+            got = cases.structured_mesh(nx, ny, nz, 5e-5, 6e-5, 7e-5, keep=keep, origin=origin, cell_grid=True)
+            ref = cases.structured_mesh_numpy(nx, ny, nz, 5e-5, 6e-5, 7e-5, keep=keep, origin=origin, cell_grid=True)
+            for a, b in zip(got, ref):
+                assert a.dtype == b.dtype and a.shape == b.shape and np.array_equal(a, b), (nx, ny, nz, origin)
+
+
+def test_crack_region_from_the_cell_grid_equals_the_full_pass():
+    """surface_crack finding its region through the cell grid gives what the pass over every
+    cell gives, for cracks inside, across the edges of and outside a holed plate and an L."""
+    rng = np.random.default_rng(5)
+    dx = 1e-4
+    plate = np.ones((30, 24, 8), dtype=bool)
+    plate[8:14, 5:11, :] = False
+    ell = np.ones((28, 28, 6), dtype=bool)
+    ell[10:, 10:, :] = False
+    for keep in (plate, ell):
+        nx, ny, nz = keep.shape
+        x, y, z, cells, ijk, grid = cases.structured_mesh_numpy(nx, ny, nz, dx, dx, dx, keep=keep, cell_grid=True)
+        lz = nz * dx
+        # centres inside, then across an edge (the region clipped), then far outside
+        spans = [(0.15, 0.85)] * 10 + [(-0.15, 0.1), (0.9, 1.15)] * 2
+        trials = [((rng.uniform(*sp) * nx * dx, rng.uniform(*sp) * ny * dx, lz), rng.uniform(0, 180),
+                   rng.uniform(2, 12) * dx, rng.uniform(1.5, 0.9 * nz) * dx) for sp in spans]
+        trials.append(((5 * nx * dx, 5 * ny * dx, lz), 30.0, 3 * dx, 2 * dx))
+        # and two that open in solid material, clear of the hole and the missing quadrant
+        trials += [((5 * dx, 5 * dx, lz), 35.0, 6 * dx, 3 * dx), ((5 * dx, 6 * dx, lz), 120.0, 8 * dx, 4 * dx)]
+        n_opened = 0
+        for centre, angle, length, depth in trials:
+            crack = cases.Crack(centre, angle, length, depth)
+            a = cases.surface_crack(x, y, z, cells, ijk, (dx, dx, dx), crack, cell_grid=grid)
+            b = cases.surface_crack(x, y, z, cells, ijk, (dx, dx, dx), crack)
+            assert a[4] == b[4]
+            n_opened += a[4] > 0
+            for u, v in zip(a[:4], b[:4]):
+                assert u.dtype == v.dtype and np.array_equal(u, v)
+        assert n_opened >= 3
