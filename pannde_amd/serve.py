@@ -230,6 +230,9 @@ class Server:
             self.class_tuning = tune_classes({size_class(r) for r in reqs}, self.precision, log=self.log)
         t_tune = time.perf_counter() - t_tune
         pool = cf.ThreadPoolExecutor(max_workers=self.prep_threads)
+        # finished solves are read back on threads of their own: those reads wait on the device,
+        # which would otherwise hold preparation threads
+        fetcher = cf.ThreadPoolExecutor(max_workers=2)
         main = rsg.Batch(self.precision, steps_per_block=self.K)
         check = rsg.Batch(rsg.RSG_FP64, steps_per_block=self.K)
         start = time.perf_counter()
@@ -383,13 +386,13 @@ class Server:
         def retire_main(slot):
             job = active.pop(slot)
             main.remove(slot)  # its last block ran one block ago: usually no wait
-            fetches[pool.submit(fetch, job, False)] = job
+            fetches[fetcher.submit(fetch, job, False)] = job
 
         def retire_check(slot):
             job = checking.pop(slot)
             check.remove(slot)
             job.slot = -1
-            fetches[pool.submit(fetch, job, True)] = job
+            fetches[fetcher.submit(fetch, job, True)] = job
 
         # the slots that finished in each of the last two enqueued blocks, older first: a solve is
         # retired one block after the block it finished in, so that its removal finds that
@@ -432,6 +435,7 @@ class Server:
         main.close()
         check.close()
         pool.shutdown()
+        fetcher.shutdown()
         ok = [r for r in reqs if results[r.rid].get("status", "ok") == "ok"]
         lat = np.array([results[r.rid]["latency"] for r in ok]) if ok else np.zeros(1)
         served_cell_steps = sum(results[r.rid]["cells"] * results[r.rid]["nsteps"] for r in ok)
