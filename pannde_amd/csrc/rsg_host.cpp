@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -1524,14 +1525,28 @@ int rsg_create_ex(const rsg_mesh_desc* mesh, const rsg_model_desc* model, int32_
   CREATE_PHASE("start");
   // links (HexMesh::buildLinks): slot = vertex ^ 6, the later cell winning a slot two cells
   // claim, as in the reference's loop
-  h->cells.assign(mesh->cell_nodes, mesh->cell_nodes + (size_t)nc * 8);
-  h->node_cells.assign((size_t)nn * 8, -1);
-  for (int32_t k = 0; k < nc; k++)
-    for (int v = 0; v < 8; v++) {
-      int32_t& slot = h->node_cells[(size_t)h->cells[(size_t)k * 8 + v] * 8 + (v ^ 6)];
-      if (slot >= 0) h->incidence_exact = false;
-      slot = k;
-    }
+  h->cells.resize((size_t)nc * 8);
+  h->node_cells.resize((size_t)nn * 8);
+  parallel_ranges(nn, [&](int64_t b, int64_t e) {
+    std::fill(h->node_cells.begin() + b * 8, h->node_cells.begin() + e * 8, -1);
+  });
+  // on threads over ranges of cells: a slot keeps the largest cell that claims it (the later
+  // one in the reference's loop), an atomic maximum, so the result is the serial loop's
+  std::atomic<bool> exact{true};
+  parallel_ranges(nc, [&](int64_t b, int64_t e) {
+    std::copy(mesh->cell_nodes + b * 8, mesh->cell_nodes + e * 8, h->cells.begin() + b * 8);
+    for (int64_t k = b; k < e; k++)
+      for (int v = 0; v < 8; v++) {
+        int32_t* slot = &h->node_cells[(size_t)h->cells[(size_t)k * 8 + v] * 8 + (v ^ 6)];
+        int32_t seen = __atomic_load_n(slot, __ATOMIC_RELAXED);
+        if (seen >= 0) exact.store(false, std::memory_order_relaxed);
+        while (seen < (int32_t)k &&
+               !__atomic_compare_exchange_n(slot, &seen, (int32_t)k, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+        }
+        if (seen >= 0) exact.store(false, std::memory_order_relaxed);
+      }
+  });
+  h->incidence_exact = exact.load();
 
   const double* X[3] = {mesh->x, mesh->y, mesh->z};
   CREATE_PHASE("links");

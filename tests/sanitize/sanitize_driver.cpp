@@ -259,6 +259,24 @@ void batches(const Box& b, const Box& small) {
 
 }  // namespace
 
+// Cells that duplicate others, appended at the end (each of their vertex slots is claimed
+// twice, on the set-up's threads): set-up, source tables with moments (whose special cells
+// then come from every cell's vertices) and a few steps, in both precisions.
+void duplicated_cells(Box b) {
+  const int32_t nc = b.nc();
+  for (int32_t k : {0, nc / 3, nc / 2, nc - 1}) {
+    for (int v = 0; v < 8; v++) b.cells.push_back(b.cells[(size_t)k * 8 + v]);
+    b.rho.push_back(b.rho[k]);
+    for (int t = 0; t < 21; t++) b.c[t].push_back(b.c[t][k]);
+  }
+  for (int precision : {RSG_FP64, RSG_FP32}) {
+    rsg_handle h = make_solver(b, precision, nullptr, 1, 3);
+    if (!h) continue;
+    EXPECT(rsg_step(h, 5, nullptr, nullptr) == RSG_OK);
+    EXPECT(rsg_destroy(h) == RSG_OK);
+  }
+}
+
 int main() {
   const Box big = make_box(64, 60, 40, 1e-4);    // > 128 Ki cells: several set-up threads
   const Box small = make_box(14, 12, 8, 1e-4);
@@ -266,6 +284,7 @@ int main() {
   single_instance(big);
   error_paths(small);
   batches(big, small);
+  duplicated_cells(big);
   const long live = stub_live_allocations();
   if (live != 0) {
     std::fprintf(stderr, "FAILED: %ld device allocations still live at exit\n", live);

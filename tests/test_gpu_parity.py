@@ -313,3 +313,34 @@ def test_jittered_planes_and_per_cell_materials():
     finally:
         s.close()
     assert_same(run_device(case), orc.run_oracle(case))
+
+
+def _with_duplicated_cells(case, ids):
+    """The case with copies of cells ``ids`` appended: each copy's vertices claim node slots
+    its original holds, and the later cell keeps the slot (HexMesh::buildLinks)."""
+    import dataclasses
+
+    ids = np.asarray(ids)
+
+    def grow(f):
+        if f.type != cases.FIELD_CELL:
+            return f
+        return cases.Field(f.type, np.concatenate([f.data, f.data[ids]]))
+
+    return dataclasses.replace(case, cells=np.concatenate([case.cells, case.cells[ids]]).astype(np.int32),
+                               rho=grow(case.rho), cij=[grow(f) for f in case.cij])
+
+
+@pytest.mark.parametrize("maker", ["water_column", "al_block"])
+def test_node_slots_claimed_twice_keep_the_later_cell(maker):
+    """Cells that duplicate others (every vertex slot claimed twice): the later cell wins
+    each slot, as in the reference's loop, and stress sources reach both copies; bitwise
+    against the oracle.  Also the path of the source tables for such meshes."""
+    if maker == "water_column":
+        case = cases.water_column_block(seed=8, nx=14, ny=12, nz=7, nsteps=150)
+    else:
+        case = cases.al_block(seed=8, nx=14, ny=12, nz=7, nsteps=150, n_sensors=6)
+    top = np.nonzero(np.isclose(case.z[case.cells].max(axis=1), case.z.max()))[0]  # cells on the top face
+    ids = list(top[len(top) // 2 - 3:len(top) // 2 + 3]) + [5, 17]  # under the source, and elsewhere
+    dup = _with_duplicated_cells(case, ids)
+    assert_same(run_device(dup), orc.run_oracle(dup))
