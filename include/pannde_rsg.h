@@ -219,8 +219,8 @@ int rsg_set_timing(rsg_handle h, int32_t enabled);
 int rsg_get_kernel_times(rsg_handle h, double* velocity_ms, double* stress_ms,
                          int64_t* launches);
 
-/* Bytes of device memory the handle holds (fields, coefficients, connectivity, source
- * and trace buffers). */
+/* Bytes of device memory the handle holds (state, coefficients, the links of each kernel:
+ * connectivity, or the LDS plan of a staged kernel, and the source and trace buffers). */
 int rsg_device_bytes(rsg_handle h, int64_t* bytes);
 
 /* Releases the handle and returns its device memory to the library's pool (one per device,
