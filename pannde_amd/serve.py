@@ -10,7 +10,8 @@ arrive as a seeded Poisson process.  The server
 * builds each admitted job and creates its solver on host threads while the GPU works;
 * folds ready solves into the one running batch at K-step boundaries (rsg_batch_add),
   where they share every launch with the solves already resident, and retires each solve
-  when its own step count is reached (its fields stay resident until then);
+  when its own step count is reached (its fields stay resident until then), reading its
+  results back off the loop that feeds the GPU;
 * re-runs a seeded fraction of the requests in FP64 (a second batch on its own stream)
   and states each checked result's relative L2 sensor-trace error against it;
 * isolates failures: a request whose preparation raises (a mesh the spacing rule rejects,
@@ -221,9 +222,10 @@ class Server:
     def serve(self, requests: List[Request], time_scale: float = 1.0) -> Dict:
         """Serves the requests (arrival times scaled by ``time_scale``) and returns the report.
 
-        The loop keeps the GPU fed: it enqueues the next K-step block of every batch before it
-        retires the solves that finished in the previous block (whose results only need that
-        block), admits and folds in newcomers, and hands solver tear-down to the pool."""
+        The loop keeps the GPU fed: it enqueues the next K-step block of every batch, then
+        removes the solves that finished two blocks back (their last block has run by then, so
+        removal does not wait), admits and folds in newcomers.  Reading a finished solve's
+        traces and energies and tearing it down happen on threads of their own."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         t_tune = time.perf_counter()
         if self.tuning == "auto":
