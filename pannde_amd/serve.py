@@ -397,8 +397,8 @@ class Server:
             fetches[fetcher.submit(fetch, job, True)] = job
 
         # the slots that finished in each of the last two enqueued blocks, older first: a solve is
-        # retired one block after the block it finished in, so that its removal finds that
-        # block done while the next one still runs
+        # retired once two more blocks have been enqueued after the one it finished in, so that
+        # its removal finds that block done while the later ones still run
         pending_main, pending_check = [[], []], [[], []]
         while st["nxt"] < len(reqs) or preps or active or checking or fetches:
             if now() - last_log >= 15.0:
