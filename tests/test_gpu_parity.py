@@ -344,3 +344,35 @@ def test_node_slots_claimed_twice_keep_the_later_cell(maker):
     ids = list(top[len(top) // 2 - 3:len(top) // 2 + 3]) + [5, 17]  # under the source, and elsewhere
     dup = _with_duplicated_cells(case, ids)
     assert_same(run_device(dup), orc.run_oracle(dup))
+
+
+@pytest.mark.parametrize("precision", [rsg.RSG_FP64, rsg.RSG_FP32])
+def test_sources_and_sensors_replaced_on_a_live_handle(precision):
+    """Table builds patch only the keys whose special flag changes: a handle whose sources and
+    sensors are replaced several times (moment discs, then force discs and other sensor nodes,
+    each change building the tables again) and then stepped equals, bit for bit, a fresh
+    handle created with the final ones."""
+    case = cases.water_column_block(seed=9, nx=16, ny=14, nz=7, nsteps=120)
+    other = cases.al_block(seed=9, nx=16, ny=14, nz=7, nsteps=120, n_sensors=9)
+    assert not np.array_equal(np.sort(case.sensors), np.sort(other.sensors))
+    live = rsg.DeviceSolver(case.x, case.y, case.z, case.cells, case.rho, case.cij, case.dt, case.force,
+                            case.moment, 1, case.sensors, precision)
+    try:
+        live.set_sources(other.force, case.moment[:1])
+        live.set_sensors(other.sensors)
+        live.set_sources(other.force, [])
+        t = live.solve(case.nsteps)
+        got = live.get_state() + (live.traces(),)
+    finally:
+        live.close()
+    fresh = rsg.DeviceSolver(case.x, case.y, case.z, case.cells, case.rho, case.cij, case.dt, other.force, [], 1,
+                             other.sensors, precision)
+    try:
+        t0 = fresh.solve(case.nsteps)
+        ref = fresh.get_state() + (fresh.traces(),)
+    finally:
+        fresh.close()
+    assert t == t0
+    assert np.abs(ref[0]).max() > 0 and np.abs(ref[2]).max() > 0
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
