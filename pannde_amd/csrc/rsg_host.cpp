@@ -322,6 +322,12 @@ struct rsg_instance {
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   bool stage_busy[2] = {false, false};
   int cur = 0;
+  // amplitudes of the library's signal tables uploaded a block of kBlockSteps steps ahead:
+  // amp[win_buf] holds those of steps [win_first, win_first + win_len), so calls of a few
+  // steps each (a driver's solve() per step) read them without a copy of their own
+  int64_t win_first = -1;
+  int32_t win_len = 0;
+  int win_buf = 0;
   DevBuf<double> trace;
   int64_t trace_dev_steps = 0;       // rows currently held on the device
   std::vector<double> trace_host;    // rows flushed to the host
@@ -1211,7 +1217,8 @@ int build_tables(rsg_instance* h) {
   } else if ((rc = copy_boxed(h, h->dc[0], dcx, cell_boxes))) {
     return rc;
   }
-  // amplitude staging for kBlockSteps steps
+  // amplitude staging for kBlockSteps steps (reallocated: no amplitudes are held ahead)
+  h->win_first = -1;
   const size_t nch = std::max<size_t>(1, h->channels.size());
   for (int b = 0; b < 2; b++) {
     if (h->stage_ev[b]) HIP_TRY(hipEventSynchronize(h->stage_ev[b]));
@@ -2030,10 +2037,21 @@ int rsg_step(rsg_handle h, int32_t nsteps, const double* amplitudes, double* tim
         return fail(RSG_ERR_ARG, "channel %zu has no signal table and no amplitudes were given", k);
   const size_t row = h->sensors.size() * 3;
   for (int32_t done = 0; done < nsteps;) {
-    const int kb = std::min(kBlockSteps, nsteps - done);
+    int kb = std::min(kBlockSteps, nsteps - done);
+    const double* amp = nch ? nullptr : h->amp[0].p;  // no channels: a buffer no term reads
+    if (nch && !amplitudes && h->win_first >= 0 && h->steps >= h->win_first &&
+        h->steps < h->win_first + h->win_len) {
+      // the next steps' amplitudes are on the device already (the signal tables are the
+      // handle's and the times follow from the same time += dt as below)
+      const int64_t off = h->steps - h->win_first;
+      kb = (int)std::min<int64_t>(kb, h->win_len - off);
+      amp = h->amp[h->win_buf].p + off * (int64_t)nch;
+    }
     if (row && h->trace_dev_steps + kb > kTraceCapSteps && (rc = flush_traces(h))) return rc;
-    const int b = h->cur;
-    if (nch) {
+    if (nch && !amp) {
+      const int b = h->cur;
+      // a table-driven upload covers a whole block ahead; caller amplitudes cover this call
+      const int up = amplitudes ? kb : kBlockSteps;
       if (h->stage_busy[b]) HIP_TRY(hipEventSynchronize(h->stage_ev[b]));
       if (!h->stage[b]) {
         HIP_TRY(hipHostMalloc((void**)&h->stage[b], nch * kBlockSteps * sizeof(double), hipHostMallocDefault));
@@ -2041,7 +2059,7 @@ int rsg_step(rsg_handle h, int32_t nsteps, const double* amplitudes, double* tim
       }
       double* st = h->stage[b];
       double t = h->time;
-      for (int k = 0; k < kb; k++) {
+      for (int k = 0; k < up; k++) {
         for (size_t c = 0; c < nch; c++) {
           double a;
           if (amplitudes) {
@@ -2056,17 +2074,24 @@ int rsg_step(rsg_handle h, int32_t nsteps, const double* amplitudes, double* tim
         }
         t += h->dt;
       }
-      HIP_TRY(hipMemcpyAsync(h->amp[b].p, st, (size_t)kb * nch * sizeof(double),
-                             hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(hipMemcpyAsync(h->amp[b].p, st, (size_t)up * nch * sizeof(double), hipMemcpyHostToDevice,
+                             h->stream));
       HIP_TRY(hipEventRecord(h->stage_ev[b], h->stream));
       h->stage_busy[b] = true;
+      h->cur ^= 1;
+      amp = h->amp[b].p;
+      if (amplitudes) {
+        h->win_first = -1;
+      } else {
+        h->win_first = h->steps;
+        h->win_len = up;
+        h->win_buf = b;
+      }
     }
-    rc = h->precision == RSG_FP64 ? run_block<double>(h, kb, h->amp[b].p)
-                                  : run_block<float>(h, kb, h->amp[b].p);
+    rc = h->precision == RSG_FP64 ? run_block<double>(h, kb, amp) : run_block<float>(h, kb, amp);
     if (rc) return rc;
     for (int k = 0; k < kb; k++) h->time += h->dt;  // HostRSGTDUT::solve: time += dt
     h->steps += kb;
-    h->cur ^= 1;
     done += kb;
   }
   if (time_out) *time_out = h->time;

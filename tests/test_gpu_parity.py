@@ -376,3 +376,31 @@ def test_sources_and_sensors_replaced_on_a_live_handle(precision):
     assert np.abs(ref[0]).max() > 0 and np.abs(ref[2]).max() > 0
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
+
+
+def test_table_amplitudes_uploaded_ahead_mix_with_caller_amplitudes():
+    """Signal-table amplitudes are uploaded a block of steps ahead and read by later calls;
+    calls with caller amplitudes in between (equal to the table's) and calls of one step at a
+    time (a driver's solve() per step) give the oracle's result bit for bit."""
+    case = cases.al_block(seed=4, nx=10, ny=9, nz=5, nsteps=90, n_sensors=3)
+    d = case.force[0]
+    t = 0.0
+    amps = []
+    for _ in range(case.nsteps):
+        amps.append(0.125 * (orc.signal_eval(d.signal_t, d.signal_v, [t])[0] * 1.0))
+        t += case.dt
+    amps = np.array(amps).reshape(-1, 1)
+    s = rsg.DeviceSolver.from_case(case)
+    try:
+        s.solve(23)
+        s.solve(4, amps[23:27])
+        for _ in range(10):
+            s.solve(1)
+        s.solve(3, amps[37:40])
+        s.solve(case.nsteps - 40)
+        V, S = s.get_state()
+        tr = s.traces()
+    finally:
+        s.close()
+    V0, S0, _, tr0 = orc.run_oracle(case)
+    assert np.array_equal(V, V0) and np.array_equal(S, S0) and np.array_equal(tr, tr0)
