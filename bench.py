@@ -375,7 +375,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded CFRP plate job generated in-process)",
         "config": {"workload": "cfrp_plate_256x256x64_dx0.05mm_16ply_5holes_rcm_2MHz", "cells": nc,
-                   "nodes": nn, "seed": args.seed, "solver": "DeviceSolver FP64 (HostRSGTDUT arithmetic)"},
+                   "nodes": nn, "seed": args.seed, "solver": "DeviceSolver FP64 (HostRSGTDUT arithmetic)",
+                   # the served legs run a bounded sample of the 1000-request stream; the whole
+                   # stream is profiled separately
+                   "served_sample": "first %d requests of the seeded 1000-request stream (seed %d)"
+                                    % (args.serve_requests, args.serve_seed)},
         "roofline": {"bound": "hbm", "achieved": round(str_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(str_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "stress_kernel<double>",
