@@ -44,6 +44,26 @@ def test_full_size_fp64_matches_oracle(workload, nsteps):
     assert np.array_equal(tr, tr0)
 
 
+@pytest.mark.timeout(900)
+def test_served_requests_fp64_match_oracle():
+    """Requests of the inspection stream as the server builds them (a cracked, holed plate in the
+    generator's own numbering and a cracked bracket, transducer and crack placed per request):
+    the FP64 DeviceSolver equals the oracle bit for bit after a few steps, source active."""
+    reqs = serve.request_stream(seed=9, n=1000, rate=4.0)[:24]
+    picks = [min((r for r in reqs if r.kind == kind), key=lambda r: r.cells) for kind in ("plate", "bracket")]
+    for r in picks:
+        case = dataclasses.replace(serve.build_case(r), nsteps=5)
+        print("%s request %d: %d cells, %d nodes, %d duplicated nodes" % (r.kind, r.rid, case.n_cells, case.n_nodes,
+                                                                          case.meta.get("n_dup", 0)), flush=True)
+        assert case.meta.get("n_dup", 0) > 0  # the crack is open
+        V, S, t, tr = device_run(case, case.nsteps)
+        V0, S0, t0, tr0 = orc.run_oracle(case, nsteps=case.nsteps)
+        assert np.abs(V0).max() > 0 and np.abs(S0).max() > 0
+        assert t == t0
+        assert np.array_equal(V, V0) and np.array_equal(S, S0)
+        assert np.array_equal(tr, tr0)
+
+
 @pytest.mark.timeout(1200)
 def test_batch_config_equals_alone_and_fp32_within_tolerance():
     inst = cases.batch_instances(seed=4)
