@@ -212,3 +212,28 @@ def test_crack_region_from_the_cell_grid_equals_the_full_pass():
             for u, v in zip(a[:4], b[:4]):
                 assert u.dtype == v.dtype and np.array_equal(u, v)
         assert n_opened >= 3
+
+
+@pytest.mark.skipif(cases._native() is None, reason="libpannde_cases.so is not built")
+def test_served_requests_built_natively_equal_numpy(monkeypatch):
+    """The served stream's smallest cracked plate and cracked bracket, generated with the
+    native mesh builder and with the numpy one: every array of the case is equal."""
+    from pannde_amd import serve
+
+    reqs = serve.request_stream(seed=9, n=1000, rate=4.0)[:24]
+    picks = [min((r for r in reqs if r.kind == kind), key=lambda r: r.cells) for kind in ("plate", "bracket")]
+
+    def arrays(c):
+        out = dict(x=c.x, y=c.y, z=c.z, cells=c.cells, rho=c.rho.data, sensors=c.sensors,
+                   dup=np.array([c.meta.get("n_dup", 0)]))
+        out.update({"c%d" % k: f.data for k, f in enumerate(c.cij)})
+        return out
+
+    for r in picks:
+        native = arrays(serve.build_case(r))
+        monkeypatch.setattr(cases, "_NATIVE", False)
+        ref = arrays(serve.build_case(r))
+        monkeypatch.setattr(cases, "_NATIVE", None)
+        assert native["dup"][0] > 0
+        for k in ref:
+            assert native[k].dtype == ref[k].dtype and np.array_equal(native[k], ref[k]), (r.kind, k)
