@@ -63,6 +63,12 @@
 #define STREAM_LOAD(p) __builtin_nontemporal_load(p)
 #endif
 
+// Velocity staging layout: 1 (default) keeps the six staged stress components as three rows
+// of pairs, so one LDS read returns two components of a cell; 0 keeps six rows.
+#ifndef RSG_VELOCITY_PAIRS
+#define RSG_VELOCITY_PAIRS 1
+#endif
+
 namespace rsg {
 namespace {
 
@@ -119,6 +125,18 @@ __device__ __forceinline__ void load8_cells(const T* __restrict__ f, const int (
                                             T (&out)[8]) {
 #pragma unroll
   for (int k = 0; k < 8; k++) out[k] = DIRECT ? ((c[k] >= 0) ? f[c[k]] : T(0)) : f[c[k]];
+}
+// The same from a staged pair row: f gets the first component of each pair, g the second.
+template <typename T>
+__device__ __forceinline__ void load8_pairs(const T* rows, const int (&c)[8], T (&f)[8], T (&g)[8]) {
+  typedef T P __attribute__((ext_vector_type(2)));
+  const P* q = reinterpret_cast<const P*>(rows);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const P v = q[c[k]];
+    f[k] = v.x;
+    g[k] = v.y;
+  }
 }
 template <typename T>
 __device__ __forceinline__ void load8_nodes(const T* __restrict__ f, const int (&c)[8],
@@ -228,15 +246,37 @@ __device__ __forceinline__ uint32_t load_lkey(const uint32_t* __restrict__ lkey,
   return (uint32_t)w.w;
 }
 
+// Staged rows: with PAIRS, components 2p and 2p + 1 of entry i sit side by side at pair p * ld
+// + i (NCOMP / 2 rows of pairs, the same bytes as NCOMP rows), so one LDS read returns both.
+template <typename T, int NCOMP, bool PAIRS>
+__device__ __forceinline__ void put_staged(T* lds, int ld, int i, const T (&v)[NCOMP]) {
+  if constexpr (PAIRS) {
+    static_assert(NCOMP % 2 == 0, "pairs");
+    typedef T P __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int p = 0; p < NCOMP / 2; p++) reinterpret_cast<P*>(lds)[p * ld + i] = P{v[2 * p], v[2 * p + 1]};
+  } else {
+#pragma unroll
+    for (int k = 0; k < NCOMP; k++) lds[k * ld + i] = v[k];
+  }
+}
+template <typename T, int NCOMP, bool PAIRS>
+__device__ __forceinline__ void zero_staged(T* lds, int ld, int zero) {
+  if (zero < 0 || (int)threadIdx.x >= NCOMP) return;
+  const int k = threadIdx.x;
+  lds[PAIRS ? 2 * ((k >> 1) * ld + zero) + (k & 1) : k * ld + zero] = T(0);
+}
+
 // Stages NCOMP fields of the block's unique gather set (ulist[u_off[b]..u_off[b+1]))
-// into LDS as [NCOMP][ld], each value read from HBM once per block.  With U > 0 every
-// thread owns at most U entries and all their loads are issued before any wait (two
-// dependent round trips in all: the list, then the values); U = 0 is the general loop.
-// zero >= 0 also stores 0 at entry `zero` of every row (the target of missing links; it
-// lies past every block's set).  With CHUNKS and `chunked` the list holds one word per 4 LDS
-// slots, start << 2 | (len - 1), and slot 4c + k takes entity start + k when k < len (the
-// other slots of the chunk stay unwritten: no link points at them).
-template <typename T, int NCOMP, int U, bool CHUNKS = false>
+// into LDS as [NCOMP][ld] (with PAIRS as [NCOMP / 2][ld] pairs), each value read from HBM
+// once per block.  With U > 0 every thread owns at most U entries and all their loads are
+// issued before any wait (two dependent round trips in all: the list, then the values);
+// U = 0 is the general loop.  zero >= 0 also stores 0 at entry `zero` of every row (the
+// target of missing links; it lies past every block's set).  With CHUNKS and `chunked` the
+// list holds one word per 4 LDS slots, start << 2 | (len - 1), and slot 4c + k takes entity
+// start + k when k < len (the other slots of the chunk stay unwritten: no link points at
+// them).
+template <typename T, int NCOMP, int U, bool CHUNKS = false, bool PAIRS = false>
 __device__ __forceinline__ void stage(T* lds, int ld, const int32_t* __restrict__ u_off,
                                       const int32_t* __restrict__ ulist, int b,
                                       const T* const (&src)[NCOMP], int zero, bool chunked = false) {
@@ -259,22 +299,20 @@ __device__ __forceinline__ void stage(T* lds, int ld, const int32_t* __restrict_
 #pragma unroll
           for (int k = 0; k < NCOMP; k++) v[u][k] = on[u] ? src[k][g[u]] : T(0);
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int i = threadIdx.x + u * blockDim.x;
-          if (on[u])
-#pragma unroll
-            for (int k = 0; k < NCOMP; k++) lds[k * ld + i] = v[u][k];
-        }
+        for (int u = 0; u < U; u++)
+          if (on[u]) put_staged<T, NCOMP, PAIRS>(lds, ld, threadIdx.x + u * blockDim.x, v[u]);
       } else {
         for (int i = threadIdx.x; i < nu; i += blockDim.x) {
           const int w = STREAM_LOAD(ulist + c0 + (i >> 2));
           if ((i & 3) > (w & 3)) continue;
           const int g = (w >> 2) + (i & 3);
+          T v[NCOMP];
 #pragma unroll
-          for (int k = 0; k < NCOMP; k++) lds[k * ld + i] = src[k][g];
+          for (int k = 0; k < NCOMP; k++) v[k] = src[k][g];
+          put_staged<T, NCOMP, PAIRS>(lds, ld, i, v);
         }
       }
-      if (zero >= 0 && (int)threadIdx.x < NCOMP) lds[threadIdx.x * ld + zero] = T(0);
+      zero_staged<T, NCOMP, PAIRS>(lds, ld, zero);
       __syncthreads();
       return;
     }
@@ -295,18 +333,18 @@ __device__ __forceinline__ void stage(T* lds, int ld, const int32_t* __restrict_
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = threadIdx.x + u * blockDim.x;
-      if (i < nu)
-#pragma unroll
-        for (int k = 0; k < NCOMP; k++) lds[k * ld + i] = v[u][k];
+      if (i < nu) put_staged<T, NCOMP, PAIRS>(lds, ld, i, v[u]);
     }
   } else {
     for (int i = threadIdx.x; i < nu; i += blockDim.x) {
       const int g = STREAM_LOAD(ulist + u0 + i);
+      T v[NCOMP];
 #pragma unroll
-      for (int k = 0; k < NCOMP; k++) lds[k * ld + i] = src[k][g];
+      for (int k = 0; k < NCOMP; k++) v[k] = src[k][g];
+      put_staged<T, NCOMP, PAIRS>(lds, ld, i, v);
     }
   }
-  if (zero >= 0 && (int)threadIdx.x < NCOMP) lds[threadIdx.x * ld + zero] = T(0);
+  zero_staged<T, NCOMP, PAIRS>(lds, ld, zero);
   __syncthreads();
 }
 
@@ -372,22 +410,40 @@ __device__ __forceinline__ void velocity_apply(const VelocityArgs<T>& a, int n, 
       idt = a.sp_idt[sp];
     }
   }
-  T f[8];
-  load8_cells<!LDS>(S0, in.c, f);
-  const T dx_sxx = dscale(diffsum<0>(f), sx);
-  load8_cells<!LDS>(S5, in.c, f);
-  const T dx_sxy = dscale(diffsum<0>(f), sx);
-  const T dy_sxy = dscale(diffsum<1>(f), sy);
-  load8_cells<!LDS>(S4, in.c, f);
-  const T dx_sxz = dscale(diffsum<0>(f), sx);
-  const T dz_sxz = dscale(diffsum<2>(f), sz);
-  load8_cells<!LDS>(S1, in.c, f);
-  const T dy_syy = dscale(diffsum<1>(f), sy);
-  load8_cells<!LDS>(S3, in.c, f);
-  const T dy_syz = dscale(diffsum<1>(f), sy);
-  const T dz_syz = dscale(diffsum<2>(f), sz);
-  load8_cells<!LDS>(S2, in.c, f);
-  const T dz_szz = dscale(diffsum<2>(f), sz);
+  T dx_sxx, dx_sxy, dy_sxy, dx_sxz, dz_sxz, dy_syy, dy_syz, dz_syz, dz_szz;
+  if constexpr (LDS && RSG_VELOCITY_PAIRS) {
+    // S0, S1, S2 are the staged pair rows (Sxx, Sxy), (Sxz, Syy), (Syz, Szz)
+    T f[8], g[8];
+    load8_pairs(S0, in.c, f, g);
+    dx_sxx = dscale(diffsum<0>(f), sx);
+    dx_sxy = dscale(diffsum<0>(g), sx);
+    dy_sxy = dscale(diffsum<1>(g), sy);
+    load8_pairs(S1, in.c, f, g);
+    dx_sxz = dscale(diffsum<0>(f), sx);
+    dz_sxz = dscale(diffsum<2>(f), sz);
+    dy_syy = dscale(diffsum<1>(g), sy);
+    load8_pairs(S2, in.c, f, g);
+    dy_syz = dscale(diffsum<1>(f), sy);
+    dz_syz = dscale(diffsum<2>(f), sz);
+    dz_szz = dscale(diffsum<2>(g), sz);
+  } else {
+    T f[8];
+    load8_cells<!LDS>(S0, in.c, f);
+    dx_sxx = dscale(diffsum<0>(f), sx);
+    load8_cells<!LDS>(S5, in.c, f);
+    dx_sxy = dscale(diffsum<0>(f), sx);
+    dy_sxy = dscale(diffsum<1>(f), sy);
+    load8_cells<!LDS>(S4, in.c, f);
+    dx_sxz = dscale(diffsum<0>(f), sx);
+    dz_sxz = dscale(diffsum<2>(f), sz);
+    load8_cells<!LDS>(S1, in.c, f);
+    dy_syy = dscale(diffsum<1>(f), sy);
+    load8_cells<!LDS>(S3, in.c, f);
+    dy_syz = dscale(diffsum<1>(f), sy);
+    dz_syz = dscale(diffsum<2>(f), sz);
+    load8_cells<!LDS>(S2, in.c, f);
+    dz_szz = dscale(diffsum<2>(f), sz);
+  }
 
   T r0 = (dx_sxx + dy_sxy) + dz_sxz;
   T r1 = (dx_sxy + dy_syy) + dz_syz;
@@ -441,16 +497,26 @@ __device__ __forceinline__ void velocity_body(const VelocityArgs<T>& a, int b, u
   const T* S5 = a.S5;
   if constexpr (LDS) {
     T* lds = reinterpret_cast<T*>(smem);
-    const T* const src[6] = {a.S0, a.S1, a.S2, a.S3, a.S4, a.S5};
     // rows of max_u + 1 entries: entry max_u is the zero that missing cells read
     const int ld = a.max_u + 1;
-    stage<T, 6, U, true>(lds, ld, a.u_off, a.ulist, b, src, a.max_u, a.uchunk != 0);
-    S0 = lds;
-    S1 = lds + ld;
-    S2 = lds + 2 * ld;
-    S3 = lds + 3 * ld;
-    S4 = lds + 4 * ld;
-    S5 = lds + 5 * ld;
+    if constexpr (RSG_VELOCITY_PAIRS) {
+      // pairs in the order the update reads them: (Sxx, Sxy), (Sxz, Syy), (Syz, Szz)
+      const T* const src[6] = {a.S0, a.S5, a.S4, a.S1, a.S3, a.S2};
+      stage<T, 6, U, true, true>(lds, ld, a.u_off, a.ulist, b, src, a.max_u, a.uchunk != 0);
+      S0 = lds;
+      S1 = lds + 2 * ld;
+      S2 = lds + 4 * ld;
+      S3 = S4 = S5 = nullptr;
+    } else {
+      const T* const src[6] = {a.S0, a.S1, a.S2, a.S3, a.S4, a.S5};
+      stage<T, 6, U, true>(lds, ld, a.u_off, a.ulist, b, src, a.max_u, a.uchunk != 0);
+      S0 = lds;
+      S1 = lds + ld;
+      S2 = lds + 2 * ld;
+      S3 = lds + 3 * ld;
+      S4 = lds + 4 * ld;
+      S5 = lds + 5 * ld;
+    }
   }
   if (base >= a.n) return;
   velocity_apply<T, LDS, KM>(a, base, in, S0, S1, S2, S3, S4, S5);
