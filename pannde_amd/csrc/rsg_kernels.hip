@@ -68,6 +68,12 @@
 #ifndef RSG_VELOCITY_PAIRS
 #define RSG_VELOCITY_PAIRS 1
 #endif
+// Stress staging layout: the three staged velocity components as a row of (Vx, Vy) pairs and
+// a row of Vz (one read fewer per vertex) in FP32 (1), in both precisions (2), or as three
+// rows (0).
+#ifndef RSG_STRESS_PAIRS
+#define RSG_STRESS_PAIRS 2
+#endif
 
 namespace rsg {
 namespace {
@@ -247,14 +253,15 @@ __device__ __forceinline__ uint32_t load_lkey(const uint32_t* __restrict__ lkey,
 }
 
 // Staged rows: with PAIRS, components 2p and 2p + 1 of entry i sit side by side at pair p * ld
-// + i (NCOMP / 2 rows of pairs, the same bytes as NCOMP rows), so one LDS read returns both.
+// + i (NCOMP / 2 rows of pairs; an odd last component keeps a row of its own after them, at
+// (NCOMP - 1) * ld + i), the same bytes as NCOMP rows, so one LDS read returns two components.
 template <typename T, int NCOMP, bool PAIRS>
 __device__ __forceinline__ void put_staged(T* lds, int ld, int i, const T (&v)[NCOMP]) {
   if constexpr (PAIRS) {
-    static_assert(NCOMP % 2 == 0, "pairs");
     typedef T P __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int p = 0; p < NCOMP / 2; p++) reinterpret_cast<P*>(lds)[p * ld + i] = P{v[2 * p], v[2 * p + 1]};
+    if constexpr (NCOMP % 2) lds[(NCOMP - 1) * ld + i] = v[NCOMP - 1];
   } else {
 #pragma unroll
     for (int k = 0; k < NCOMP; k++) lds[k * ld + i] = v[k];
@@ -264,7 +271,7 @@ template <typename T, int NCOMP, bool PAIRS>
 __device__ __forceinline__ void zero_staged(T* lds, int ld, int zero) {
   if (zero < 0 || (int)threadIdx.x >= NCOMP) return;
   const int k = threadIdx.x;
-  lds[PAIRS ? 2 * ((k >> 1) * ld + zero) + (k & 1) : k * ld + zero] = T(0);
+  lds[PAIRS && k < NCOMP - NCOMP % 2 ? 2 * ((k >> 1) * ld + zero) + (k & 1) : k * ld + zero] = T(0);
 }
 
 // Stages NCOMP fields of the block's unique gather set (ulist[u_off[b]..u_off[b+1]))
@@ -607,19 +614,32 @@ __device__ __forceinline__ void voigt_update(T dt, R cc, T d00, T d11, T d22, T 
 
 // Strain-rate terms of one cell from its 8 vertices: the diagonal D_i V_i and the shear
 // sums D_j V_k + D_k V_j, in the reference's order of operations.
-template <typename T>
+// With PAIRS, V0 is the staged row of (Vx, Vy) pairs and V1 is unused.
+template <typename T, bool PAIRS = false>
 __device__ __forceinline__ void strain_terms(const T* V0, const T* V1, const T* V2, const int (&c)[8], T sx,
                                              T sy, T sz, T (&d)[6]) {
   // delV[i][k] = D_i V_k (i: derivative axis, k: velocity component)
   T f[8];
-  load8_nodes(V0, c, f);
-  const T d00 = dscale(diffsum<0>(f), sx);
-  const T d10 = dscale(diffsum<1>(f), sy);
-  const T d20 = dscale(diffsum<2>(f), sz);
-  load8_nodes(V1, c, f);
-  const T d01 = dscale(diffsum<0>(f), sx);
-  const T d11 = dscale(diffsum<1>(f), sy);
-  const T d21 = dscale(diffsum<2>(f), sz);
+  T d00, d10, d20, d01, d11, d21;
+  if constexpr (PAIRS) {
+    T g[8];
+    load8_pairs(V0, c, f, g);
+    d00 = dscale(diffsum<0>(f), sx);
+    d10 = dscale(diffsum<1>(f), sy);
+    d20 = dscale(diffsum<2>(f), sz);
+    d01 = dscale(diffsum<0>(g), sx);
+    d11 = dscale(diffsum<1>(g), sy);
+    d21 = dscale(diffsum<2>(g), sz);
+  } else {
+    load8_nodes(V0, c, f);
+    d00 = dscale(diffsum<0>(f), sx);
+    d10 = dscale(diffsum<1>(f), sy);
+    d20 = dscale(diffsum<2>(f), sz);
+    load8_nodes(V1, c, f);
+    d01 = dscale(diffsum<0>(f), sx);
+    d11 = dscale(diffsum<1>(f), sy);
+    d21 = dscale(diffsum<2>(f), sz);
+  }
   load8_nodes(V2, c, f);
   const T d02 = dscale(diffsum<0>(f), sx);
   const T d12 = dscale(diffsum<1>(f), sy);
@@ -666,8 +686,9 @@ __device__ __forceinline__ void voigt_update(T dt, R cc, const T (&d)[6], const 
   }
 }
 
-// S += dt*(C : grad V + m) at cell e, V gathered from V0..V2 (LDS or HBM).
-template <typename T, int KM>
+// S += dt*(C : grad V + m) at cell e, V gathered from V0..V2 (LDS or HBM; with PAIRS, V0 is
+// the staged row of (Vx, Vy) pairs).
+template <typename T, int KM, bool PAIRS = false>
 __device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrIn<T>& in, const T* V0,
                                              const T* V1, const T* V2) {
   T d[6];
@@ -695,7 +716,7 @@ __device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrI
       T c0[21];
 #pragma unroll
       for (int j = 0; j < 21; j++) c0[j] = row0[j];
-      strain_terms(V0, V1, V2, in.c, sx, sy, sz, d);
+      strain_terms<T, PAIRS>(V0, V1, V2, in.c, sx, sy, sz, d);
       cell_moments(a, sp, m);
       if (__all(mat == m0)) {
         voigt_update(a.dt, c0, d, m, sp, in.s);
@@ -722,7 +743,7 @@ __device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrI
 #pragma unroll
         for (int j = 0; j < 21; j++) cc[j] = row[j];
       }
-      strain_terms(V0, V1, V2, in.c, sx, sy, sz, d);
+      strain_terms<T, PAIRS>(V0, V1, V2, in.c, sx, sy, sz, d);
       cell_moments(a, sp, m);
       voigt_update(a.dt, cc, d, m, sp, in.s);
     }
@@ -733,7 +754,7 @@ __device__ __forceinline__ void stress_apply(const StressArgs<T>& a, int e, StrI
       sp = unbox(sx);
       sx = a.sp_dcx[sp];
     }
-    strain_terms(V0, V1, V2, in.c, sx, in.sy, in.sz, d);
+    strain_terms<T, PAIRS>(V0, V1, V2, in.c, sx, in.sy, in.sz, d);
     cell_moments(a, sp, m);
     voigt_update(a.dt, in.cc, d, m, sp, in.s);
   }
@@ -759,16 +780,17 @@ __device__ __forceinline__ void stress_body(const StressArgs<T>& a, int b, unsig
   const T* V0 = a.V0;
   const T* V1 = a.V1;
   const T* V2 = a.V2;
+  constexpr bool PAIRS = LDS && (RSG_STRESS_PAIRS == 2 || (RSG_STRESS_PAIRS == 1 && sizeof(T) == 4));
   if constexpr (LDS) {
     T* lds = reinterpret_cast<T*>(smem);
     const T* const src[3] = {a.V0, a.V1, a.V2};
-    stage<T, 3, U>(lds, a.max_u, a.u_off, a.ulist, b, src, -1);
+    stage<T, 3, U, false, PAIRS>(lds, a.max_u, a.u_off, a.ulist, b, src, -1);
     V0 = lds;
-    V1 = lds + a.max_u;
+    V1 = PAIRS ? nullptr : lds + a.max_u;
     V2 = lds + 2 * a.max_u;
   }
   if (base >= a.n) return;
-  stress_apply<T, KM>(a, base, in, V0, V1, V2);
+  stress_apply<T, KM, PAIRS>(a, base, in, V0, V1, V2);
   // later cells: each cell's streaming operands are requested one cell ahead
 #pragma unroll 1
   for (int j = 1; j < CPT; j++) {
@@ -776,7 +798,7 @@ __device__ __forceinline__ void stress_body(const StressArgs<T>& a, int b, unsig
     if (e >= a.n) break;
     in = next;
     if (j + 1 < CPT) stress_load<T, LDS, KM>(a, e + bd < a.n ? e + bd : a.n - 1, next);
-    stress_apply<T, KM>(a, e, in, V0, V1, V2);
+    stress_apply<T, KM, PAIRS>(a, e, in, V0, V1, V2);
   }
 }
 
