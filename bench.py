@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 #   velocity: V read+write 3x8x2 + dt/rho 8 + spacings 3x8 + cell links 8x4 + S gathered 6x8 = 160
 #   FP32: stress 188, velocity 96.  This layout moves fewer bytes than that model, so the model is
 #   reported separately and the roofline uses the layout's own algorithmic bytes.
-SURVEY_MODEL_BYTES = {"stress": {"f64": 344, "f32": 188}, "velocity": {"f64": 160, "f32": 96}}
+REFERENCE_MODEL_BYTES = {"stress": {"f64": 344, "f32": 188}, "velocity": {"f64": 160, "f32": 96}}
 # Every 32nd step's two launches are bracketed by HIP events inside the timed region (about 60
 # samples per kernel in a 2000-step run): events between every launch slowed the stream by 4 %
 # (FP64) to 8 % (FP32).
@@ -358,8 +358,8 @@ def main():
               "links": {2: "10-bit block-local packed with the key (LDS staging)",
                         1: "16-bit block-local (LDS staging)", 0: "32-bit global"}[tun["links_stress"]],
               "cells_per_thread_stress": tun["per_thread_stress"]}
-    model_str = SURVEY_MODEL_BYTES["stress"]["f64"] * nc
-    model_vel = SURVEY_MODEL_BYTES["velocity"]["f64"] * nn
+    model_str = REFERENCE_MODEL_BYTES["stress"]["f64"] * nc
+    model_vel = REFERENCE_MODEL_BYTES["velocity"]["f64"] * nn
 
     out = {
         "metric": "Gcell-steps/s FP64 vs FP32 @<=1e-4 trace err; served solves/hr + p50/p99 lat",
